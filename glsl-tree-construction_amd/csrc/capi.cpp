@@ -1,0 +1,552 @@
+// capi.cpp -- context, scratch management, the plain pipeline and the
+// host-array mirrors of the reference's per-stage Rust functions.
+//
+// The reference creates a Vulkan instance, device and queue inside every stage
+// call and round-trips each stage's arrays through host-visible buffers
+// (e.g. src/morton.rs:84-168,227-357).  Here one context owns the device
+// scratch; device-pointer entry points are stream-ordered; the host-array
+// mirrors copy in, run, copy out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "tc_internal.hpp"
+
+namespace {
+
+constexpr int FILL_BLOCK = 256;
+
+__global__ __launch_bounds__(FILL_BLOCK) void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    const size_t stride = (size_t)gridDim.x * FILL_BLOCK;
+    for (size_t i = (size_t)blockIdx.x * FILL_BLOCK + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        return hipMalloc(&p, bytes) == hipSuccess ? TC_OK : TC_ERR_NOMEM;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+int set_device(tc_ctx* ctx) { return hipSetDevice(ctx->device) == hipSuccess ? TC_OK : TC_ERR_HIP; }
+
+}  // namespace
+
+int tc_fill_u32(tc_ctx* ctx, uint32_t* ptr, uint32_t value, size_t count, hipStream_t s) {
+    if (count == 0) return TC_OK;
+    size_t blocks = (count + FILL_BLOCK - 1) / FILL_BLOCK;
+    const size_t cap = (size_t)ctx->num_cus * 8;
+    if (blocks > cap) blocks = cap;
+    ProfScope prof(ctx, s, K_FILL);
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(FILL_BLOCK), 0, s, ptr, value, count);
+    TC_LAUNCH_CHECK();
+    return TC_OK;
+}
+
+int tc_ensure_scratch(tc_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->scratch_bytes) return TC_OK;
+    if (ctx->scratch) {
+        // Earlier work may still use the old block.
+        TC_HIP(hipDeviceSynchronize());
+        TC_HIP(hipFree(ctx->scratch));
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    want = (want + 4095) & ~(size_t)4095;
+    if (hipMalloc(&ctx->scratch, want) != hipSuccess) {
+        ctx->scratch = nullptr;
+        return TC_ERR_NOMEM;
+    }
+    ctx->scratch_bytes = want;
+    return TC_OK;
+}
+
+extern "C" const char* tc_status_string(int status) {
+    switch (status) {
+        case TC_OK: return "ok";
+        case TC_ERR_INVALID: return "invalid argument";
+        case TC_ERR_NOMEM: return "out of memory";
+        case TC_ERR_TOO_LARGE: return "size above the 2^30 limit";
+        case TC_ERR_HIP: return "HIP runtime error";
+        case TC_ERR_OVER_BUDGET: return "over the memory budget";
+        case TC_ERR_CAPACITY: return "output capacity too small";
+        case TC_ERR_BUSY: return "busy";
+        case TC_ERR_NOT_FOUND: return "not found";
+        case TC_ERR_NO_DEVICE: return "no usable GPU";
+        default: return "unknown status";
+    }
+}
+
+extern "C" const char* tc_version(void) { return "tc 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------------ profiler
+hipEvent_t tc_prof_event(tc_ctx* ctx) {
+    if (!ctx->prof_pool.empty()) {
+        hipEvent_t e = ctx->prof_pool.back();
+        ctx->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+extern "C" const char* tc_kernel_name(int id) {
+    static const char* names[K_COUNT] = {"morton", "histogram", "binning", "unique", "scan", "radix_tree",
+                                         "edge_count", "edge_count_scan", "octree_emit", "octree_link",
+                                         "fill", "decode", "segmented"};
+    return (id >= 0 && id < K_COUNT) ? names[id] : "?";
+}
+
+extern "C" int tc_profile_enable(tc_ctx* ctx, int on) {
+    if (!ctx) return TC_ERR_INVALID;
+    ctx->prof_on = on != 0;
+    return TC_OK;
+}
+
+extern "C" int tc_profile_collect(tc_ctx* ctx) {
+    if (!ctx) return TC_ERR_INVALID;
+    for (ProfRec& r : ctx->prof_pending) {
+        if (hipEventSynchronize(r.b) != hipSuccess) return TC_ERR_HIP;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            ctx->prof_ms[r.id] += ms;
+            ctx->prof_n[r.id] += 1;
+        }
+        ctx->prof_pool.push_back(r.a);
+        ctx->prof_pool.push_back(r.b);
+    }
+    ctx->prof_pending.clear();
+    return TC_OK;
+}
+
+extern "C" int tc_profile_reset(tc_ctx* ctx) {
+    if (!ctx) return TC_ERR_INVALID;
+    TC_TRY(tc_profile_collect(ctx));
+    for (int i = 0; i < K_COUNT; ++i) {
+        ctx->prof_ms[i] = 0.0;
+        ctx->prof_n[i] = 0;
+    }
+    return TC_OK;
+}
+
+extern "C" int tc_profile_get(tc_ctx* ctx, int id, double* total_ms, uint64_t* launches) {
+    if (!ctx || !total_ms || !launches || id < 0 || id >= K_COUNT) return TC_ERR_INVALID;
+    *total_ms = ctx->prof_ms[id];
+    *launches = ctx->prof_n[id];
+    return TC_OK;
+}
+
+extern "C" int tc_create(tc_ctx** out, int device) {
+    if (!out) return TC_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return TC_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return TC_ERR_INVALID;
+    tc_ctx* ctx = new (std::nothrow) tc_ctx();
+    if (!ctx) return TC_ERR_NOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete ctx;
+        return TC_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->num_cus = prop.multiProcessorCount;
+    const size_t status_bytes = ((size_t)(1u << 18) + 1) * sizeof(uint64_t);
+    if (hipMalloc(&ctx->small, SMALL_WORDS * 4) != hipSuccess ||
+        hipMalloc(&ctx->scan_status, status_bytes) != hipSuccess ||
+        hipMemset(ctx->small, 0, SMALL_WORDS * 4) != hipSuccess ||
+        hipMemset(ctx->scan_status, 0, status_bytes) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        tc_destroy(ctx);
+        return TC_ERR_NOMEM;
+    }
+    *out = ctx;
+    return TC_OK;
+}
+
+extern "C" int tc_destroy(tc_ctx* ctx) {
+    if (!ctx) return TC_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    (void)tc_profile_collect(ctx);
+    for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->small) (void)hipFree(ctx->small);
+    if (ctx->scan_status) (void)hipFree(ctx->scan_status);
+    delete ctx;
+    return TC_OK;
+}
+
+extern "C" int tc_reserve(tc_ctx* ctx, uint32_t max_n) {
+    if (!ctx) return TC_ERR_INVALID;
+    if (max_n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    TC_TRY(set_device(ctx));
+    // Smallest tile of any supported launch shape (256 x 8) needs the most words.
+    return tc_ensure_scratch(ctx, tc_sort_scratch_bytes(max_n, 8, 256 * 8));
+}
+
+extern "C" int tc_set_sort_params(tc_ctx* ctx, int block, int kpt) {
+    if (!ctx) return TC_ERR_INVALID;
+    if (block == 0) block = ctx->sort_block;
+    if (kpt == 0) kpt = ctx->sort_kpt;
+    if (block != 256 && block != 512) return TC_ERR_INVALID;
+    if (kpt != 8 && kpt != 12 && kpt != 16 && kpt != 20) return TC_ERR_INVALID;
+    ctx->sort_block = block;
+    ctx->sort_kpt = kpt;
+    return TC_OK;
+}
+
+extern "C" int tc_get_sort_params(tc_ctx* ctx, int* block, int* kpt) {
+    if (!ctx || !block || !kpt) return TC_ERR_INVALID;
+    *block = ctx->sort_block;
+    *kpt = ctx->sort_kpt;
+    return TC_OK;
+}
+
+extern "C" uint64_t tc_octree_node_bound(uint64_t n_keys, int code_bits) {
+    if (n_keys < 2) return 0;
+    const uint64_t half = n_keys / 2;
+    uint64_t total = 1, p = 1;
+    const int levels = (code_bits - 1) / 3;
+    for (int L = 1; L <= levels; ++L) {
+        p = p < (1ull << 60) ? p * 8 : p;
+        total += std::min(p, half);
+    }
+    return total;
+}
+
+// --------------------------------------------------------------- host decode
+namespace {
+uint32_t h_compact3_10(uint32_t m) {
+    uint32_t x = m & 0x09249249u;
+    x = (x ^ (x >> 2)) & 0x030c30c3u;
+    x = (x ^ (x >> 4)) & 0x0300f00fu;
+    x = (x ^ (x >> 8)) & 0x030000ffu;
+    x = (x ^ (x >> 16)) & 0x000003ffu;
+    return x;
+}
+uint64_t h_compact3_21(uint64_t m) {
+    uint64_t x = m & 0x1249249249249249ull;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x ^ (x >> 8)) & 0x001f0000ff0000ffull;
+    x = (x ^ (x >> 16)) & 0x001f00000000ffffull;
+    x = (x ^ (x >> 32)) & 0x00000000001fffffull;
+    return x;
+}
+}  // namespace
+
+// morton32_to_xyz, src/morton.rs:43-59 (volatile keeps each fp32 step rounded).
+extern "C" void tc_morton32_to_xyz(float ret[4], uint32_t code, float min_coord, float range) {
+    for (int a = 0; a < 3; ++a) {
+        volatile float f = (float)h_compact3_10(code >> a);
+        f = f / 1024.0f;
+        f = f * range;
+        ret[a] = f + min_coord;
+    }
+    ret[3] = 1.0f;
+}
+
+extern "C" void tc_morton64_to_xyz(float ret[4], uint64_t code, float min_coord, float range) {
+    for (int a = 0; a < 3; ++a) {
+        volatile float f = (float)h_compact3_21(code >> a);
+        f = f / 2097152.0f;
+        f = f * range;
+        ret[a] = f + min_coord;
+    }
+    ret[3] = 1.0f;
+}
+
+// ------------------------------------------------------------ plain pipeline
+extern "C" int tc_tree_alloc(tc_ctx* ctx, uint32_t capacity, int code_bits, tc_tree* t) {
+    if (!ctx || !t || (code_bits != 30 && code_bits != 63)) return TC_ERR_INVALID;
+    if (capacity > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    TC_TRY(set_device(ctx));
+    std::memset(t, 0, sizeof(*t));
+    t->capacity = capacity;
+    t->code_bits = code_bits;
+    t->node_capacity = tc_octree_node_bound(capacity, code_bits);
+    if (t->node_capacity > 0xFFFFFFFFull) t->node_capacity = 0xFFFFFFFFull;
+    const size_t n = capacity ? capacity : 1;
+    const size_t ks = code_bits == 30 ? 4 : 8;
+    bool ok = hipMalloc(&t->sorted, n * ks) == hipSuccess && hipMalloc(&t->alt, n * ks) == hipSuccess &&
+              hipMalloc(&t->unique, n * ks) == hipSuccess &&
+              hipMalloc((void**)&t->prefix_n, n) == hipSuccess &&
+              hipMalloc((void**)&t->has_leaf_left, n) == hipSuccess &&
+              hipMalloc((void**)&t->has_leaf_right, n) == hipSuccess &&
+              hipMalloc((void**)&t->left_child, n * 4) == hipSuccess &&
+              hipMalloc((void**)&t->parent, n * 4) == hipSuccess &&
+              hipMalloc((void**)&t->edge_count, n * 4) == hipSuccess &&
+              hipMalloc((void**)&t->offsets, (n + 1) * 4) == hipSuccess &&
+              hipMalloc((void**)&t->nodes, std::max<uint64_t>(t->node_capacity, 1) * sizeof(tc_oct_node)) ==
+                  hipSuccess &&
+              hipMalloc((void**)&t->d_counts, 16) == hipSuccess;
+    if (!ok) {
+        tc_tree_free(ctx, t);
+        return TC_ERR_NOMEM;
+    }
+    return tc_reserve(ctx, capacity);
+}
+
+extern "C" int tc_tree_free(tc_ctx* ctx, tc_tree* t) {
+    if (!ctx || !t) return TC_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    void* ptrs[] = {t->sorted, t->alt, t->unique, t->prefix_n, t->has_leaf_left, t->has_leaf_right,
+                    t->left_child, t->parent, t->edge_count, t->offsets, t->nodes, t->d_counts};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    std::memset(t, 0, sizeof(*t));
+    return TC_OK;
+}
+
+extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float min_coord, float range,
+                             int link_leaves, tc_tree* t, tc_stream stream) {
+    if (!ctx || !t || (n && !pts)) return TC_ERR_INVALID;
+    if (n > t->capacity) return TC_ERR_CAPACITY;
+    hipStream_t s = as_stream(stream);
+    t->n_points = n;
+    // Counts start at zero so that an empty cloud reads back as empty.
+    TC_TRY(tc_fill_u32(ctx, t->d_counts, 0u, 4, s));
+    if (n == 0) return TC_OK;
+    const uint32_t node_cap = (uint32_t)std::min<uint64_t>(t->node_capacity, 0xFFFFFFFFull);
+    if (t->code_bits == 30) {
+        uint32_t* sorted = static_cast<uint32_t*>(t->sorted);
+        uint32_t* uniq = static_cast<uint32_t*>(t->unique);
+        TC_TRY(tc_morton30(ctx, pts, sorted, n, min_coord, range, stream));
+        TC_TRY(tc_sort_keys_u32(ctx, sorted, static_cast<uint32_t*>(t->alt), n, stream));
+        TC_TRY(tc_unique_u32(ctx, sorted, uniq, n, t->d_counts, stream));
+        TC_TRY(tc_build_radix_tree_u32(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
+                                       t->has_leaf_right, t->left_child, t->parent, stream));
+        if (n >= 2) {
+            TC_TRY(tc_edge_count_scan(ctx, t->prefix_n, t->parent, t->edge_count, t->offsets, n - 1, t->d_counts,
+                                      stream));
+            TC_TRY(tc_build_octree_u32(ctx, t->nodes, node_cap, t->offsets, t->edge_count, uniq, t->prefix_n,
+                                       t->parent, t->left_child, t->has_leaf_left, t->has_leaf_right, n - 1,
+                                       t->d_counts, min_coord, range, link_leaves, stream));
+        }
+    } else {
+        uint64_t* sorted = static_cast<uint64_t*>(t->sorted);
+        uint64_t* uniq = static_cast<uint64_t*>(t->unique);
+        TC_TRY(tc_morton63(ctx, pts, sorted, n, min_coord, range, stream));
+        TC_TRY(tc_sort_keys_u64(ctx, sorted, static_cast<uint64_t*>(t->alt), n, stream));
+        TC_TRY(tc_unique_u64(ctx, sorted, uniq, n, t->d_counts, stream));
+        TC_TRY(tc_build_radix_tree_u64(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
+                                       t->has_leaf_right, t->left_child, t->parent, stream));
+        if (n >= 2) {
+            TC_TRY(tc_edge_count_scan(ctx, t->prefix_n, t->parent, t->edge_count, t->offsets, n - 1, t->d_counts,
+                                      stream));
+            TC_TRY(tc_build_octree_u64(ctx, t->nodes, node_cap, t->offsets, t->edge_count, uniq, t->prefix_n,
+                                       t->parent, t->left_child, t->has_leaf_left, t->has_leaf_right, n - 1,
+                                       t->d_counts, min_coord, range, link_leaves, stream));
+        }
+    }
+    return TC_OK;
+}
+
+extern "C" int tc_tree_read_counts(tc_ctx* ctx, tc_tree* t, tc_stream stream) {
+    if (!ctx || !t) return TC_ERR_INVALID;
+    hipStream_t s = as_stream(stream);
+    TC_HIP(hipStreamSynchronize(s));
+    uint32_t u = 0;
+    TC_HIP(hipMemcpy(&u, t->d_counts, 4, hipMemcpyDeviceToHost));
+    t->n_unique = u;
+    t->n_brt = u > 1 ? u - 1 : 0;
+    t->n_nodes = 0;
+    if (t->n_brt > 0) TC_HIP(hipMemcpy(&t->n_nodes, t->offsets + t->n_brt, 4, hipMemcpyDeviceToHost));
+    if (t->n_nodes > t->node_capacity) return TC_ERR_CAPACITY;
+    return TC_OK;
+}
+
+// ------------------------------------------------------ host-array mirrors
+// Each mirror follows the argument order of the cited Rust function; the
+// reference's group_size (its dispatch width) is accepted and ignored because
+// the launch shape is chosen per kernel here.
+namespace {
+int h2d(void* d, const void* h, size_t bytes) {
+    if (!bytes) return TC_OK;
+    return hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) == hipSuccess ? TC_OK : TC_ERR_HIP;
+}
+int d2h(void* h, const void* d, size_t bytes) {
+    if (!bytes) return TC_OK;
+    return hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost) == hipSuccess ? TC_OK : TC_ERR_HIP;
+}
+int sync0() { return hipDeviceSynchronize() == hipSuccess ? TC_OK : TC_ERR_HIP; }
+}  // namespace
+
+// compute_morton, src/morton.rs:76-83.
+extern "C" int tc_host_compute_morton(tc_ctx* ctx, const float* input, uint32_t* keys, uint32_t n, float min_coord,
+                                      float range, uint32_t group_size) {
+    (void)group_size;
+    if (!ctx || (n && (!input || !keys))) return TC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TC_TRY(set_device(ctx));
+    DevBuf dp, dk;
+    TC_TRY(dp.alloc((size_t)n * 16));
+    TC_TRY(dk.alloc((size_t)n * 4));
+    TC_TRY(h2d(dp.p, input, (size_t)n * 16));
+    TC_TRY(tc_morton30(ctx, dp.as<float>(), dk.as<uint32_t>(), n, min_coord, range, nullptr));
+    TC_TRY(sync0());
+    return d2h(keys, dk.p, (size_t)n * 4);
+}
+
+// histogram, src/histogram.rs:38 (global_hist holds 4 x 256 words).
+extern "C" int tc_host_histogram(tc_ctx* ctx, uint32_t input_size, const uint32_t* input, uint32_t* global_hist) {
+    if (!ctx || !global_hist || (input_size && !input)) return TC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TC_TRY(set_device(ctx));
+    DevBuf dk, dh;
+    TC_TRY(dk.alloc((size_t)input_size * 4));
+    TC_TRY(dh.alloc(4 * 256 * 4));
+    TC_TRY(h2d(dk.p, input, (size_t)input_size * 4));
+    TC_TRY(tc_global_histogram_u32(ctx, dk.as<uint32_t>(), input_size, dh.as<uint32_t>(), nullptr));
+    TC_TRY(sync0());
+    return d2h(global_hist, dh.p, 4 * 256 * 4);
+}
+
+// radix_sort, src/radixsort.rs:36 (the pipeline's contract: ascending u32 keys).
+extern "C" int tc_host_radix_sort(tc_ctx* ctx, const uint32_t* in, uint32_t n, uint32_t* out, uint32_t group_size) {
+    (void)group_size;
+    if (!ctx || (n && (!in || !out))) return TC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TC_TRY(set_device(ctx));
+    DevBuf dk, da;
+    TC_TRY(dk.alloc((size_t)n * 4));
+    TC_TRY(da.alloc((size_t)n * 4));
+    TC_TRY(h2d(dk.p, in, (size_t)n * 4));
+    TC_TRY(tc_sort_keys_u32(ctx, dk.as<uint32_t>(), da.as<uint32_t>(), n, nullptr));
+    TC_TRY(sync0());
+    return d2h(out, dk.p, (size_t)n * 4);
+}
+
+// unique, src/unique.rs:37: u_sort[0..U) receives the unique keys, U returned.
+extern "C" int tc_host_unique(tc_ctx* ctx, uint32_t* u_sort, uint32_t n, uint32_t* n_unique) {
+    if (!ctx || !n_unique || (n && !u_sort)) return TC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TC_TRY(set_device(ctx));
+    DevBuf dk, dout, dc;
+    TC_TRY(dk.alloc((size_t)n * 4));
+    TC_TRY(dout.alloc((size_t)n * 4));
+    TC_TRY(dc.alloc(4));
+    TC_TRY(h2d(dk.p, u_sort, (size_t)n * 4));
+    TC_TRY(tc_unique_u32(ctx, dk.as<uint32_t>(), dout.as<uint32_t>(), n, dc.as<uint32_t>(), nullptr));
+    TC_TRY(sync0());
+    TC_TRY(d2h(n_unique, dc.p, 4));
+    return d2h(u_sort, dout.p, (size_t)*n_unique * 4);
+}
+
+// build_radix_tree, src/build_radix_tree.rs:34-43: n is the node count
+// (n_unique - 1, as src/main.rs:106-107 passes); codes holds n + 1 keys.
+extern "C" int tc_host_build_radix_tree(tc_ctx* ctx, int32_t n, const uint32_t* codes, uint8_t* prefix_n,
+                                        uint8_t* hll, uint8_t* hlr, int32_t* left, int32_t* parent,
+                                        uint32_t group_size) {
+    (void)group_size;
+    if (!ctx || n < 0) return TC_ERR_INVALID;
+    if (n == 0) return TC_OK;
+    if (!codes || !prefix_n || !hll || !hlr || !left || !parent) return TC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TC_TRY(set_device(ctx));
+    const size_t nb = (size_t)n, nk = nb + 1;
+    DevBuf dc, dp, dl, dr, dlc, dpa;
+    TC_TRY(dc.alloc(nk * 4));
+    TC_TRY(dp.alloc(nb));
+    TC_TRY(dl.alloc(nb));
+    TC_TRY(dr.alloc(nb));
+    TC_TRY(dlc.alloc(nb * 4));
+    TC_TRY(dpa.alloc(nb * 4));
+    TC_TRY(h2d(dc.p, codes, nk * 4));
+    TC_TRY(tc_build_radix_tree_u32(ctx, dc.as<uint32_t>(), (uint32_t)nk, nullptr, dp.as<uint8_t>(), dl.as<uint8_t>(),
+                                   dr.as<uint8_t>(), dlc.as<int32_t>(), dpa.as<int32_t>(), nullptr));
+    TC_TRY(sync0());
+    TC_TRY(d2h(prefix_n, dp.p, nb));
+    TC_TRY(d2h(hll, dl.p, nb));
+    TC_TRY(d2h(hlr, dr.p, nb));
+    TC_TRY(d2h(left, dlc.p, nb * 4));
+    return d2h(parent, dpa.p, nb * 4);
+}
+
+// edge_count, src/edge_count.rs:41-47: n_brt_nodes radix nodes (n_unique - 1).
+extern "C" int tc_host_edge_count(tc_ctx* ctx, const uint8_t* prefix_n, const int32_t* parent, uint32_t* ec,
+                                  uint32_t n_brt, uint32_t group_size) {
+    (void)group_size;
+    if (!ctx || (n_brt && (!prefix_n || !parent || !ec))) return TC_ERR_INVALID;
+    if (n_brt == 0) return TC_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TC_TRY(set_device(ctx));
+    DevBuf dp, dpa, de;
+    TC_TRY(dp.alloc(n_brt));
+    TC_TRY(dpa.alloc((size_t)n_brt * 4));
+    TC_TRY(de.alloc((size_t)n_brt * 4));
+    TC_TRY(h2d(dp.p, prefix_n, n_brt));
+    TC_TRY(h2d(dpa.p, parent, (size_t)n_brt * 4));
+    TC_TRY(tc_edge_count(ctx, dp.as<uint8_t>(), dpa.as<int32_t>(), de.as<uint32_t>(), n_brt, nullptr, nullptr));
+    TC_TRY(sync0());
+    return d2h(ec, de.p, (size_t)n_brt * 4);
+}
+
+// partial_sum, src/partial_sum.rs:43 with the fix-up of src/main.rs:127-129:
+// d_first[0 .. last-first] = exclusive scan of u_edge_count[first .. last).
+extern "C" int tc_host_partial_sum(tc_ctx* ctx, const uint32_t* in, uint32_t first, uint32_t last, uint32_t* d_first) {
+    if (!ctx || !d_first || last < first || (last > first && !in)) return TC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TC_TRY(set_device(ctx));
+    const size_t n = last - first;
+    DevBuf di, dout;
+    TC_TRY(di.alloc(n * 4));
+    TC_TRY(dout.alloc((n + 1) * 4));
+    TC_TRY(h2d(di.p, in + first, n * 4));
+    TC_TRY(tc_exclusive_scan_u32(ctx, di.as<uint32_t>(), dout.as<uint32_t>(), (uint32_t)n, nullptr, nullptr));
+    TC_TRY(sync0());
+    return d2h(d_first, dout.p, (n + 1) * 4);
+}
+
+// build_octree, src/octree.rs:73-87: n radix nodes; oct_nodes must hold
+// node_offsets[n] entries (the root included, at index 0).
+extern "C" int tc_host_build_octree(tc_ctx* ctx, tc_oct_node* oct_nodes, const uint32_t* offsets,
+                                    const uint32_t* counts, const uint32_t* codes, const uint8_t* prefix_n,
+                                    const int32_t* parents, float min_coord, float range, int32_t n,
+                                    const uint8_t* hll, const uint8_t* hlr, const int32_t* left,
+                                    uint32_t group_size) {
+    (void)group_size;
+    if (!ctx || n < 0) return TC_ERR_INVALID;
+    if (n == 0) return TC_OK;
+    if (!oct_nodes || !offsets || !counts || !codes || !prefix_n || !parents || !hll || !hlr || !left)
+        return TC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TC_TRY(set_device(ctx));
+    const size_t nb = (size_t)n;
+    const uint32_t total = offsets[nb];
+    DevBuf dn, doff, dcnt, dcode, dpn, dpar, dl, dr, dlc;
+    TC_TRY(dn.alloc((size_t)total * sizeof(tc_oct_node)));
+    TC_TRY(doff.alloc((nb + 1) * 4));
+    TC_TRY(dcnt.alloc(nb * 4));
+    TC_TRY(dcode.alloc((nb + 1) * 4));
+    TC_TRY(dpn.alloc(nb));
+    TC_TRY(dpar.alloc(nb * 4));
+    TC_TRY(dl.alloc(nb));
+    TC_TRY(dr.alloc(nb));
+    TC_TRY(dlc.alloc(nb * 4));
+    TC_TRY(h2d(doff.p, offsets, (nb + 1) * 4));
+    TC_TRY(h2d(dcnt.p, counts, nb * 4));
+    TC_TRY(h2d(dcode.p, codes, (nb + 1) * 4));
+    TC_TRY(h2d(dpn.p, prefix_n, nb));
+    TC_TRY(h2d(dpar.p, parents, nb * 4));
+    TC_TRY(h2d(dl.p, hll, nb));
+    TC_TRY(h2d(dr.p, hlr, nb));
+    TC_TRY(h2d(dlc.p, left, nb * 4));
+    TC_TRY(tc_build_octree_u32(ctx, dn.as<tc_oct_node>(), total, doff.as<uint32_t>(), dcnt.as<uint32_t>(),
+                               dcode.as<uint32_t>(), dpn.as<uint8_t>(), dpar.as<int32_t>(), dlc.as<int32_t>(),
+                               dl.as<uint8_t>(), dr.as<uint8_t>(), (uint32_t)nb, nullptr, min_coord, range, 1,
+                               nullptr));
+    TC_TRY(sync0());
+    return d2h(oct_nodes, dn.p, (size_t)total * sizeof(tc_oct_node));
+}

@@ -1,0 +1,445 @@
+// sort.hip -- one-pass global histogram + OneSweep LSD digit-binning passes.
+//
+// Reference: k_GlobalHistogram, shader/histogram.comp:103-195 and
+// k_DigitBinning, shader/new_radix_sort.comp:116-314, driven by
+// test_radix_sort, src/main.rs:185-800.  Published algorithm: Adinets &
+// Merrill, "Onesweep: A Faster Least Significant Digit Radix Sort for GPUs"
+// (2022), with Merrill & Garland's decoupled look-back.
+//
+// gfx950 design:
+//  * histogram: one read of the keys (16 B vector loads) builds all 4 (u32) or
+//    8 (u64) digit histograms in LDS sub-histograms, merged with device atomics;
+//    the last block to finish turns the counts into exclusive prefixes and
+//    clears the accumulator for the next call.  It also clears the pass-0
+//    look-back words and the per-pass tile counters.
+//  * binning pass: a tile of BLOCK*KPT keys per workgroup, tile ids handed out
+//    by an atomic counter (forward progress for the look-back).  Each wave
+//    ranks its 64*KPT keys with 64-lane ballots (8 ballots give the peer mask
+//    of lanes holding the same digit), so the rank is stable.  Per-wave digit
+//    counters live in LDS; a block scan turns them into tile-local offsets; keys
+//    are staged in LDS in digit order and written out in runs (coalesced
+//    stores).  Look-back words are u32 (2-bit flag, 30-bit count) published and
+//    read with agent-scope atomics, since the 8 XCDs' L2 caches are not
+//    coherent with each other; one thread per digit walks back.  Each pass
+//    clears the other look-back region for the next pass.
+#include "tc_internal.hpp"
+
+namespace {
+
+constexpr int HIST_BLOCK = 256;
+constexpr int HIST_REP = 2;  // LDS sub-histogram copies per block (waves alternate)
+
+template <typename K> struct KeyVec;
+template <> struct KeyVec<uint32_t> { using V = uint4; static constexpr int N = 4; };
+template <> struct KeyVec<uint64_t> { using V = ulonglong2; static constexpr int N = 2; };
+
+template <typename K>
+TC_DEV void vec_get(const typename KeyVec<K>::V& v, K* out);
+template <>
+TC_DEV void vec_get<uint32_t>(const uint4& v, uint32_t* out) {
+    out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+}
+template <>
+TC_DEV void vec_get<uint64_t>(const ulonglong2& v, uint64_t* out) {
+    out[0] = v.x; out[1] = v.y;
+}
+
+template <typename K>
+__global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
+    const K* __restrict__ keys, uint32_t n, uint32_t* __restrict__ acc, uint32_t* __restrict__ done,
+    uint32_t* __restrict__ hist_out, uint32_t* __restrict__ zero_ptr, uint32_t zero_words) {
+    constexpr int DIG = (int)sizeof(K);
+    constexpr int VN = KeyVec<K>::N;
+    __shared__ uint32_t s_h[HIST_REP][DIG * 256];
+    __shared__ uint32_t s_tmp[HIST_BLOCK / 64 + 2];
+    __shared__ uint32_t s_last;
+
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < HIST_REP * DIG * 256; i += HIST_BLOCK) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+
+    uint32_t* h = s_h[(tid / 64) % HIST_REP];
+    const uint32_t gtid = blockIdx.x * HIST_BLOCK + tid;
+    const uint32_t gstride = gridDim.x * HIST_BLOCK;
+    const uint32_t nvec = n / VN;
+    const typename KeyVec<K>::V* kv = reinterpret_cast<const typename KeyVec<K>::V*>(keys);
+    for (uint32_t v = gtid; v < nvec; v += gstride) {
+        K k[VN];
+        vec_get<K>(kv[v], k);
+#pragma unroll
+        for (int j = 0; j < VN; ++j)
+#pragma unroll
+            for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((k[j] >> (8 * d)) & 255u)], 1u);
+    }
+    for (uint32_t i = nvec * VN + gtid; i < n; i += gstride) {
+        const K k = keys[i];
+#pragma unroll
+        for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((k >> (8 * d)) & 255u)], 1u);
+    }
+    // Clear the pass-0 look-back words and the tile counters for the passes.
+    for (uint32_t w = gtid; w < zero_words; w += gstride) zero_ptr[w] = 0u;
+    __syncthreads();
+
+    for (uint32_t b = tid; b < DIG * 256; b += HIST_BLOCK) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int r = 0; r < HIST_REP; ++r) s += s_h[r][b];
+        if (s) atomicAdd(&acc[b], s);
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1u : 0u;
+    __syncthreads();
+    if (!s_last) return;
+
+    // Last block: counts -> exclusive prefixes (digit-major), then clear.
+    __threadfence();
+    for (int d = 0; d < DIG; ++d) {
+        const uint32_t c = ld_acquire_agent(&acc[d * 256 + tid]);
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<HIST_BLOCK>(c, s_tmp, &tot);
+        hist_out[d * 256 + tid] = ex;
+        st_agent(&acc[d * 256 + tid], 0u);
+    }
+    if (tid == 0) st_agent(done, 0u);
+}
+
+// Peer mask: lanes of this wave whose digit equals mine, among `live` lanes.
+TC_DEV uint64_t match_digit8(uint32_t d, uint64_t live) {
+    uint64_t peers = live;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+template <typename K>
+TC_DEV uint32_t digit_of(K k, uint32_t shift) {
+    return (uint32_t)((k >> shift) & 255u);
+}
+
+template <typename K, bool PAIRS, int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_onesweep(
+    const K* __restrict__ sk, K* __restrict__ dk, const uint32_t* __restrict__ sv,
+    uint32_t* __restrict__ dv, uint32_t n, uint32_t shift, const uint32_t* __restrict__ digit_prefix,
+    uint32_t* __restrict__ lb_cur, uint32_t* __restrict__ lb_next, uint32_t* __restrict__ tile_ctr) {
+    constexpr int WAVES = BLOCK / 64;
+    constexpr uint32_t TILE = BLOCK * KPT;
+    constexpr uint32_t WAVE_TILE = 64 * KPT;
+    static_assert(BLOCK >= 256, "one thread per digit");
+    constexpr size_t STAGE_BYTES = TILE * sizeof(K);
+
+    __shared__ uint32_t s_whist[WAVES * 256];
+    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_gbase[256];
+    __shared__ uint32_t s_tmp[WAVES + 2];
+    __shared__ uint32_t s_tile;
+    __shared__ __attribute__((aligned(16))) unsigned char s_stage[STAGE_BYTES];
+    K* s_keys = reinterpret_cast<K*>(s_stage);
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_stage);  // reused after the keys are out
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+
+    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_whist[i] = 0u;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * TILE;
+    const bool full = base + TILE <= n;
+    const uint32_t wbase = base + wave * WAVE_TILE + lane;
+
+    K keys[KPT];
+    uint32_t vals[KPT];
+    uint32_t pos[KPT];
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) keys[i] = sk[wbase + i * 64];
+        if (PAIRS) {
+#pragma unroll
+            for (int i = 0; i < KPT; ++i) vals[i] = sv[wbase + i * 64];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) {
+            const uint32_t idx = wbase + i * 64;
+            keys[i] = idx < n ? sk[idx] : (K)0;
+            if (PAIRS) vals[i] = idx < n ? sv[idx] : 0u;
+        }
+    }
+    if (lb_next) {
+        for (uint32_t i = tid; i < 256; i += BLOCK) lb_next[tile * 256 + i] = 0u;
+    }
+
+    // Stable wave-level ranking.  Lanes of one wave touch this wave's counters
+    // only; LDS operations of a wave complete in order, so each iteration sees
+    // the previous iteration's leader updates.
+    volatile uint32_t* wh = s_whist + wave * 256;
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const uint32_t d = digit_of<K>(keys[i], shift);
+        const bool valid = full || (wbase + i * 64 < n);
+        const uint64_t peers = match_digit8(d, __ballot(valid));
+        const uint32_t lt = mbcnt64(peers);
+        const uint32_t before = wh[d];
+        pos[i] = before + lt;
+        if (valid && lt == 0) wh[d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+
+    // Per digit: exclusive prefix over waves, tile total, publish aggregate.
+    uint32_t total_d = 0;
+    if (tid < 256) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = s_whist[w * 256 + tid];
+            s_whist[w * 256 + tid] = run;
+            run += c;
+        }
+        total_d = run;
+        st_agent(&lb_cur[tile * 256 + tid], (tile == 0 ? LB_INCLUSIVE : LB_AGGREGATE) | (total_d << 2));
+    }
+    uint32_t tile_total;
+    const uint32_t start = block_exclusive_scan<BLOCK>(total_d, s_tmp, &tile_total);
+    if (tid < 256) s_start[tid] = start;
+    __syncthreads();
+
+    // Stage keys in LDS in (digit, original order) order.
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const bool valid = full || (wbase + i * 64 < n);
+        if (valid) {
+            const uint32_t d = digit_of<K>(keys[i], shift);
+            const uint32_t lp = s_start[d] + s_whist[wave * 256 + d] + pos[i];
+            s_keys[lp] = keys[i];
+            pos[i] = lp;
+        }
+    }
+
+    // Decoupled look-back, one thread per digit.
+    if (tid < 256) {
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int32_t k = (int32_t)tile - 1;
+            while (true) {
+                const uint32_t v = ld_acquire_agent(&lb_cur[(uint32_t)k * 256 + tid]);
+                const uint32_t f = v & LB_FLAG_MASK;
+                if (f == LB_NOT_READY) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v >> 2;
+                if (f == LB_INCLUSIVE) break;
+                --k;
+            }
+            st_agent(&lb_cur[tile * 256 + tid], LB_INCLUSIVE | ((excl + total_d) << 2));
+        }
+        s_gbase[tid] = digit_prefix[tid] + excl - start;
+    }
+    __syncthreads();
+
+    const uint32_t cnt = full ? TILE : n - base;
+    uint32_t dst[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const uint32_t j = tid + i * BLOCK;
+        if (j < cnt) {
+            const K key = s_keys[j];
+            dst[i] = s_gbase[digit_of<K>(key, shift)] + j;
+            dk[dst[i]] = key;
+        }
+    }
+    if (PAIRS) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) {
+            const bool valid = full || (wbase + i * 64 < n);
+            if (valid) s_vals[pos[i]] = vals[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) {
+            const uint32_t j = tid + i * BLOCK;
+            if (j < cnt) dv[dst[i]] = s_vals[j];
+        }
+    }
+}
+
+template <typename K, bool PAIRS, int BLOCK, int KPT>
+void launch_pass(uint32_t tiles, hipStream_t s, const K* sk, K* dk, const uint32_t* sv, uint32_t* dv,
+                 uint32_t n, uint32_t shift, const uint32_t* prefix, uint32_t* lb_cur, uint32_t* lb_next,
+                 uint32_t* ctr) {
+    hipLaunchKernelGGL((k_onesweep<K, PAIRS, BLOCK, KPT>), dim3(tiles), dim3(BLOCK), 0, s, sk, dk, sv, dv,
+                       n, shift, prefix, lb_cur, lb_next, ctr);
+}
+
+template <typename K, bool PAIRS>
+int dispatch_pass(tc_ctx* ctx, uint32_t tiles, hipStream_t s, const K* sk, K* dk, const uint32_t* sv,
+                  uint32_t* dv, uint32_t n, uint32_t shift, const uint32_t* prefix, uint32_t* lb_cur,
+                  uint32_t* lb_next, uint32_t* ctr) {
+    ProfScope prof(ctx, s, K_BINNING);
+#define TC_PASS_CASE(B, KP)                                                                         \
+    if (ctx->sort_block == B && ctx->sort_kpt == KP) {                                              \
+        launch_pass<K, PAIRS, B, KP>(tiles, s, sk, dk, sv, dv, n, shift, prefix, lb_cur, lb_next, ctr); \
+        TC_LAUNCH_CHECK();                                                                           \
+        return TC_OK;                                                                                \
+    }
+    TC_PASS_CASE(256, 8)
+    TC_PASS_CASE(256, 12)
+    TC_PASS_CASE(256, 16)
+    TC_PASS_CASE(256, 20)
+    TC_PASS_CASE(512, 8)
+    TC_PASS_CASE(512, 12)
+    TC_PASS_CASE(512, 16)
+    TC_PASS_CASE(512, 20)
+#undef TC_PASS_CASE
+    return TC_ERR_INVALID;
+}
+
+uint32_t hist_grid(tc_ctx* ctx, uint32_t n) {
+    uint32_t blocks = ceil_div_u32(n, HIST_BLOCK * 16);
+    uint32_t cap = (uint32_t)ctx->num_cus * 2;
+    if (blocks < 1) blocks = 1;
+    return blocks < cap ? blocks : cap;
+}
+
+// Scratch layout for a sort: [16 tile counters][region 0][region 1][hist]
+struct SortScratch {
+    uint32_t* ctr;
+    uint32_t* region[2];
+    uint32_t* hist;
+};
+
+int sort_scratch(tc_ctx* ctx, uint32_t n, int digits, SortScratch* ss, uint32_t* tiles_out) {
+    const uint32_t tile = (uint32_t)(ctx->sort_block * ctx->sort_kpt);
+    const uint32_t tiles = ceil_div_u32(n, tile);
+    const size_t region_words = (size_t)tiles * 256;
+    const size_t words = 64 + 2 * region_words + (size_t)digits * 256;
+    TC_TRY(tc_ensure_scratch(ctx, words * 4));
+    uint32_t* base = static_cast<uint32_t*>(ctx->scratch);
+    ss->ctr = base;
+    ss->region[0] = base + 64;
+    ss->region[1] = base + 64 + region_words;
+    ss->hist = base + 64 + 2 * region_words;
+    *tiles_out = tiles;
+    return TC_OK;
+}
+
+template <typename K>
+int histogram_impl(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist_out, uint32_t* zero_ptr,
+                   uint32_t zero_words, hipStream_t s) {
+    ProfScope prof(ctx, s, K_HIST);
+    hipLaunchKernelGGL((k_global_histogram<K>), dim3(hist_grid(ctx, n)), dim3(HIST_BLOCK), 0, s, keys, n,
+                       ctx->small + SMALL_HIST_ACC, ctx->small + SMALL_HIST_DONE, hist_out, zero_ptr,
+                       zero_words);
+    TC_LAUNCH_CHECK();
+    return TC_OK;
+}
+
+template <typename K>
+int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint32_t n, hipStream_t s) {
+    if (!ctx || (n && (!keys || !alt))) return TC_ERR_INVALID;
+    if (vals && !valt) return TC_ERR_INVALID;
+    if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (n == 0) return TC_OK;
+    if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
+    constexpr int DIG = (int)sizeof(K);
+    SortScratch ss;
+    uint32_t tiles;
+    TC_TRY(sort_scratch(ctx, n, DIG, &ss, &tiles));
+    const uint32_t region_words = tiles * 256;
+    // Histogram also clears the tile counters and look-back region 0.
+    TC_TRY(histogram_impl<K>(ctx, keys, n, ss.hist, ss.ctr, 64 + region_words, s));
+    K* src = keys;
+    K* dst = alt;
+    uint32_t* vsrc = vals;
+    uint32_t* vdst = valt;
+    for (int p = 0; p < DIG; ++p) {
+        uint32_t* cur = ss.region[p & 1];
+        uint32_t* nxt = (p + 1 < DIG) ? ss.region[(p + 1) & 1] : nullptr;
+        if (vals) {
+            TC_TRY((dispatch_pass<K, true>(ctx, tiles, s, src, dst, vsrc, vdst, n, 8u * p, ss.hist + 256 * p,
+                                           cur, nxt, ss.ctr + p)));
+        } else {
+            TC_TRY((dispatch_pass<K, false>(ctx, tiles, s, src, dst, nullptr, nullptr, n, 8u * p,
+                                            ss.hist + 256 * p, cur, nxt, ss.ctr + p)));
+        }
+        K* t = src; src = dst; dst = t;
+        uint32_t* tv = vsrc; vsrc = vdst; vdst = tv;
+    }
+    // An even number of passes leaves the result in keys/vals.
+    return TC_OK;
+}
+
+template <typename K>
+int binning_pass_impl(tc_ctx* ctx, const K* sk, K* dk, const uint32_t* sv, uint32_t* dv, uint32_t n,
+                      const uint32_t* hist, uint32_t shift, hipStream_t s) {
+    if (!ctx || (n && (!sk || !dk || !hist))) return TC_ERR_INVALID;
+    if ((sv == nullptr) != (dv == nullptr)) return TC_ERR_INVALID;
+    if (shift % 8 != 0 || shift >= 8 * sizeof(K)) return TC_ERR_INVALID;
+    if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (n == 0) return TC_OK;
+    SortScratch ss;
+    uint32_t tiles;
+    TC_TRY(sort_scratch(ctx, n, (int)sizeof(K), &ss, &tiles));
+    TC_TRY(tc_fill_u32(ctx, ss.ctr, 0u, 64 + (size_t)tiles * 256, s));
+    const uint32_t* prefix = hist + 256 * (shift / 8);
+    if (sv) return dispatch_pass<K, true>(ctx, tiles, s, sk, dk, sv, dv, n, shift, prefix, ss.region[0],
+                                          nullptr, ss.ctr);
+    return dispatch_pass<K, false>(ctx, tiles, s, sk, dk, nullptr, nullptr, n, shift, prefix, ss.region[0],
+                                   nullptr, ss.ctr);
+}
+
+template <typename K>
+int global_histogram_api(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist, hipStream_t s) {
+    if (!ctx || !hist || (n && !keys)) return TC_ERR_INVALID;
+    if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
+    return histogram_impl<K>(ctx, keys, n, hist, nullptr, 0, s);
+}
+
+}  // namespace
+
+size_t tc_sort_scratch_bytes(uint32_t n, int passes, int tile) {
+    const size_t tiles = ceil_div_u32(n, (uint32_t)tile);
+    return (64 + 2 * tiles * 256 + (size_t)passes * 256) * 4;
+}
+
+extern "C" int tc_global_histogram_u32(tc_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_t* hist,
+                                       tc_stream stream) {
+    return global_histogram_api<uint32_t>(ctx, keys, n, hist, as_stream(stream));
+}
+extern "C" int tc_global_histogram_u64(tc_ctx* ctx, const uint64_t* keys, uint32_t n, uint32_t* hist,
+                                       tc_stream stream) {
+    return global_histogram_api<uint64_t>(ctx, keys, n, hist, as_stream(stream));
+}
+extern "C" int tc_binning_pass_u32(tc_ctx* ctx, const uint32_t* sk, uint32_t* dk, const uint32_t* sv,
+                                   uint32_t* dv, uint32_t n, const uint32_t* hist, uint32_t shift,
+                                   tc_stream stream) {
+    return binning_pass_impl<uint32_t>(ctx, sk, dk, sv, dv, n, hist, shift, as_stream(stream));
+}
+extern "C" int tc_binning_pass_u64(tc_ctx* ctx, const uint64_t* sk, uint64_t* dk, const uint32_t* sv,
+                                   uint32_t* dv, uint32_t n, const uint32_t* hist, uint32_t shift,
+                                   tc_stream stream) {
+    return binning_pass_impl<uint64_t>(ctx, sk, dk, sv, dv, n, hist, shift, as_stream(stream));
+}
+extern "C" int tc_sort_keys_u32(tc_ctx* ctx, uint32_t* keys, uint32_t* alt, uint32_t n, tc_stream stream) {
+    return sort_impl<uint32_t>(ctx, keys, nullptr, alt, nullptr, n, as_stream(stream));
+}
+extern "C" int tc_sort_pairs_u32(tc_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt, uint32_t* valt,
+                                 uint32_t n, tc_stream stream) {
+    if (!vals) return TC_ERR_INVALID;
+    return sort_impl<uint32_t>(ctx, keys, vals, alt, valt, n, as_stream(stream));
+}
+extern "C" int tc_sort_keys_u64(tc_ctx* ctx, uint64_t* keys, uint64_t* alt, uint32_t n, tc_stream stream) {
+    return sort_impl<uint64_t>(ctx, keys, nullptr, alt, nullptr, n, as_stream(stream));
+}
+extern "C" int tc_sort_pairs_u64(tc_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* alt, uint32_t* valt,
+                                 uint32_t n, tc_stream stream) {
+    if (!vals) return TC_ERR_INVALID;
+    return sort_impl<uint64_t>(ctx, keys, vals, alt, valt, n, as_stream(stream));
+}

@@ -1,0 +1,281 @@
+// tc_internal.hpp -- shared definitions of libtc: context, scratch layout,
+// error plumbing and wave64 device helpers for gfx950 (CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/tc.h"
+
+#define TC_HIP(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return TC_ERR_HIP;       \
+    } while (0)
+
+#define TC_TRY(call)                                   \
+    do {                                               \
+        int s_ = (call);                               \
+        if (s_ != TC_OK) return s_;                    \
+    } while (0)
+
+#define TC_LAUNCH_CHECK() TC_HIP(hipGetLastError())
+
+// Lookback flag words of the sort: bits 0-1 flag, bits 2-31 count.
+constexpr uint32_t LB_NOT_READY = 0u;
+constexpr uint32_t LB_AGGREGATE = 1u;
+constexpr uint32_t LB_INCLUSIVE = 2u;
+constexpr uint32_t LB_FLAG_MASK = 3u;
+
+// Scan tile status words: bits 0-31 value, bits 32-33 flag.
+constexpr uint64_t ST_AGGREGATE = 1ull << 32;
+constexpr uint64_t ST_INCLUSIVE = 2ull << 32;
+constexpr uint64_t ST_FLAG_MASK = 3ull << 32;
+
+// Kernel ids for the per-launch event profiler (tc_profile_*).
+enum TcKernelId {
+    K_MORTON = 0,
+    K_HIST,
+    K_BINNING,
+    K_UNIQUE,
+    K_SCAN,
+    K_BRT,
+    K_EDGE,
+    K_ECSCAN,
+    K_OCT_EMIT,
+    K_OCT_LINK,
+    K_FILL,
+    K_DECODE,
+    K_SEG,  // segmented (batched) kernels
+    K_COUNT
+};
+
+struct ProfRec {
+    int id;
+    hipEvent_t a, b;
+};
+
+struct tc_ctx {
+    int device = 0;
+    // Event profiler: when on, every launch is bracketed by two events on its
+    // stream; tc_profile_collect folds the elapsed times into per-kernel sums.
+    bool prof_on = false;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    double prof_ms[K_COUNT] = {};
+    uint64_t prof_n[K_COUNT] = {};
+    int num_cus = 256;
+    // Sort launch parameters.
+    int sort_block = 256;
+    int sort_kpt = 16;
+    // Grow-only device scratch.
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // Permanent small device block: histogram accumulators (self-cleaning),
+    // completion counters, pipeline counters.
+    uint32_t* small = nullptr;
+    // Scan tile status words (epoch-tagged, never cleared): one per 4096-element
+    // tile up to 2^30 elements.
+    uint64_t* scan_status = nullptr;
+    // Host-side staging for the host-array wrappers.
+    std::mutex mu;
+};
+
+// Layout of ctx->small (u32 words).
+constexpr size_t SMALL_HIST_ACC = 0;        // 8*256 digit counters, zero between uses
+constexpr size_t SMALL_HIST_DONE = 2048;    // completion ticket of the histogram
+constexpr size_t SMALL_TILE_CTR = 2056;     // 8 per-pass tile counters (+ 8 spare)
+constexpr size_t SMALL_SCAN_CTR = 2072;     // scan tile counters (8)
+constexpr size_t SMALL_WORDS = 4096;
+
+// Scratch sizing helpers (bytes).
+int tc_ensure_scratch(tc_ctx* ctx, size_t bytes);
+size_t tc_sort_scratch_bytes(uint32_t n, int passes, int tile);
+size_t tc_scan_scratch_bytes(uint32_t n);
+
+static inline hipStream_t as_stream(tc_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Brackets one kernel launch with profiler events (no-op when profiling is off).
+hipEvent_t tc_prof_event(tc_ctx* ctx);
+struct ProfScope {
+    tc_ctx* ctx;
+    hipStream_t s;
+    int id;
+    hipEvent_t a = nullptr;
+    ProfScope(tc_ctx* c, hipStream_t st, int kid) : ctx(c), s(st), id(kid) {
+        if (ctx->prof_on) {
+            a = tc_prof_event(ctx);
+            if (a) (void)hipEventRecord(a, s);
+        }
+    }
+    ~ProfScope() {
+        if (a) {
+            hipEvent_t b = tc_prof_event(ctx);
+            if (b) {
+                (void)hipEventRecord(b, s);
+                ctx->prof_pending.push_back({id, a, b});
+            }
+        }
+    }
+};
+
+__host__ __device__ static inline uint32_t ceil_div_u32(uint64_t a, uint64_t b) {
+    return (uint32_t)((a + b - 1) / b);
+}
+
+// Entry points shared between translation units.
+int tc_fill_u32(tc_ctx* ctx, uint32_t* ptr, uint32_t value, size_t count, hipStream_t s);
+int tc_launch_scan_tiles(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0,
+                         void* out1, uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys,
+                         hipStream_t s);
+int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block);
+
+// ---------------------------------------------------------------- device side
+#ifdef __HIP_DEVICE_COMPILE__
+#define TC_DEV __device__ __forceinline__
+#else
+#define TC_DEV __device__ __forceinline__
+#endif
+
+TC_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// popcount(mask & lanes-below-me), the wave64 exclusive bit count.
+TC_DEV uint32_t mbcnt64(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+TC_DEV uint32_t ld_acquire_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+TC_DEV void st_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+TC_DEV uint64_t ld_agent64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+TC_DEV void st_agent64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Inclusive scan across the 64 lanes of a wave.
+TC_DEV uint32_t wave_inclusive_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= (uint32_t)off) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; every thread of the block
+// must call it.  s_tmp holds >= BLOCK/64 words.  Returns the exclusive prefix;
+// *total receives the block sum.
+template <int BLOCK>
+TC_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+    constexpr int WAVES = BLOCK / 64;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64;
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) s_tmp[wave] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+        uint32_t t = s_tmp[w];
+        wave_off += ((uint32_t)w < wave) ? t : 0u;
+        sum += t;
+    }
+    __syncthreads();
+    *total = sum;
+    return wave_off + inc - v;
+}
+
+// ---------------------------------------------------------- Morton helpers
+// morton3D_SplitBy3bits, shader/morton.comp:36-44.
+TC_DEV uint32_t d_split3_10(uint32_t b) {
+    uint32_t x = b & 0x000003ffu;
+    x = (x | x << 16) & 0x030000ffu;
+    x = (x | x << 8) & 0x0300f00fu;
+    x = (x | x << 4) & 0x030c30c3u;
+    x = (x | x << 2) & 0x09249249u;
+    return x;
+}
+TC_DEV uint64_t d_split3_21(uint64_t b) {
+    uint64_t x = b & 0x1fffffull;
+    x = (x | x << 32) & 0x001f00000000ffffull;
+    x = (x | x << 16) & 0x001f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+// morton3_d_get_third_bits, src/morton.rs:61-68.
+TC_DEV uint32_t d_compact3_10(uint32_t m) {
+    uint32_t x = m & 0x09249249u;
+    x = (x ^ (x >> 2)) & 0x030c30c3u;
+    x = (x ^ (x >> 4)) & 0x0300f00fu;
+    x = (x ^ (x >> 8)) & 0x030000ffu;
+    x = (x ^ (x >> 16)) & 0x000003ffu;
+    return x;
+}
+TC_DEV uint64_t d_compact3_21(uint64_t m) {
+    uint64_t x = m & 0x1249249249249249ull;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x ^ (x >> 8)) & 0x001f0000ff0000ffull;
+    x = (x ^ (x >> 16)) & 0x001f00000000ffffull;
+    x = (x ^ (x >> 32)) & 0x00000000001fffffull;
+    return x;
+}
+
+// GLSL uint(float) with out-of-range input made well defined: NaN and
+// negatives give 0, values >= 2^32 give 0xFFFFFFFF, otherwise truncation.
+TC_DEV uint32_t d_f2u_sat(float t) {
+    if (!(t > 0.0f)) return 0u;
+    if (t >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)t;
+}
+
+// ((v - min) / range) * scale with separately rounded fp32 steps
+// (single_point_to_code_v2, shader/morton.comp:50-56).
+TC_DEV uint32_t d_quantize(float v, float min_coord, float range, float scale) {
+    float t = __fsub_rn(v, min_coord);
+    t = __fdiv_rn(t, range);
+    t = __fmul_rn(t, scale);
+    return d_f2u_sat(t);
+}
+
+// Code traits: 30-bit codes in u32, 63-bit codes in u64.
+template <typename K> struct CodeTraits;
+template <> struct CodeTraits<uint32_t> {
+    static constexpr int BITS = 30;
+    TC_DEV static int delta_bits(uint32_t x) { return x ? __clz(x) - 2 : 30; }
+    TC_DEV static void decode(uint32_t code, float min_coord, float range, float* out) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float f = (float)d_compact3_10(code >> a);
+            f = __fdiv_rn(f, 1024.0f);
+            f = __fmul_rn(f, range);
+            out[a] = __fadd_rn(f, min_coord);
+        }
+        out[3] = 1.0f;
+    }
+};
+template <> struct CodeTraits<uint64_t> {
+    static constexpr int BITS = 63;
+    TC_DEV static int delta_bits(uint64_t x) { return x ? __clzll(x) - 1 : 63; }
+    TC_DEV static void decode(uint64_t code, float min_coord, float range, float* out) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float f = (float)d_compact3_21(code >> a);
+            f = __fdiv_rn(f, 2097152.0f);
+            f = __fmul_rn(f, range);
+            out[a] = __fadd_rn(f, min_coord);
+        }
+        out[3] = 1.0f;
+    }
+};

@@ -1,0 +1,169 @@
+"""ctypes bindings of libtc.so (the C ABI declared in include/tc.h).
+
+The library is built in-tree (``make`` in the package directory, or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there is
+no CPU fallback for any stage.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libtc.so")
+
+TC_OK = 0
+STATUS_NAMES = {
+    0: "TC_OK", -1: "TC_ERR_INVALID", -2: "TC_ERR_NOMEM", -3: "TC_ERR_TOO_LARGE", -4: "TC_ERR_HIP",
+    -5: "TC_ERR_OVER_BUDGET", -6: "TC_ERR_CAPACITY", -7: "TC_ERR_BUSY", -8: "TC_ERR_NOT_FOUND",
+    -9: "TC_ERR_NO_DEVICE",
+}
+TC_ERR_INVALID, TC_ERR_NOMEM, TC_ERR_TOO_LARGE, TC_ERR_HIP = -1, -2, -3, -4
+TC_ERR_OVER_BUDGET, TC_ERR_CAPACITY, TC_ERR_BUSY, TC_ERR_NOT_FOUND, TC_ERR_NO_DEVICE = -5, -6, -7, -8, -9
+
+OCT_NODE_DTYPE = np.dtype(
+    [
+        ("children", "<i4", (8,)),
+        ("corner", "<f4", (4,)),
+        ("cell_size", "<f4"),
+        ("child_node_mask", "<i4"),
+        ("child_leaf_mask", "<i4"),
+        ("pad_", "<i4"),
+    ]
+)
+assert OCT_NODE_DTYPE.itemsize == 64
+
+
+class TcError(RuntimeError):
+    def __init__(self, status: int, where: str = ""):
+        self.status = status
+        name = STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{where}: {name}" if where else name)
+
+
+class TcTree(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_uint32),
+        ("code_bits", C.c_int),
+        ("node_capacity", C.c_uint64),
+        ("sorted", C.c_void_p),
+        ("alt", C.c_void_p),
+        ("unique", C.c_void_p),
+        ("prefix_n", C.c_void_p),
+        ("has_leaf_left", C.c_void_p),
+        ("has_leaf_right", C.c_void_p),
+        ("left_child", C.c_void_p),
+        ("parent", C.c_void_p),
+        ("edge_count", C.c_void_p),
+        ("offsets", C.c_void_p),
+        ("nodes", C.c_void_p),
+        ("d_counts", C.c_void_p),
+        ("n_points", C.c_uint32),
+        ("n_unique", C.c_uint32),
+        ("n_brt", C.c_uint32),
+        ("n_nodes", C.c_uint32),
+    ]
+
+
+P = C.c_void_p
+I = C.c_int
+U32 = C.c_uint32
+U64 = C.c_uint64
+F = C.c_float
+S = C.c_void_p  # stream
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "tc_status_string": (C.c_char_p, [I]),
+    "tc_version": (C.c_char_p, []),
+    "tc_create": (I, [C.POINTER(C.c_void_p), I]),
+    "tc_destroy": (I, [P]),
+    "tc_reserve": (I, [P, U32]),
+    "tc_set_sort_params": (I, [P, I, I]),
+    "tc_profile_enable": (I, [P, I]),
+    "tc_profile_collect": (I, [P]),
+    "tc_profile_reset": (I, [P]),
+    "tc_profile_get": (I, [P, I, C.POINTER(C.c_double), C.POINTER(U64)]),
+    "tc_kernel_name": (C.c_char_p, [I]),
+    "tc_get_sort_params": (I, [P, C.POINTER(I), C.POINTER(I)]),
+    "tc_morton30": (I, [P, P, P, U32, F, F, S]),
+    "tc_morton63": (I, [P, P, P, U32, F, F, S]),
+    "tc_morton30_decode": (I, [P, P, P, U32, F, F, S]),
+    "tc_morton32_to_xyz": (None, [P, U32, F, F]),
+    "tc_morton64_to_xyz": (None, [P, U64, F, F]),
+    "tc_global_histogram_u32": (I, [P, P, U32, P, S]),
+    "tc_global_histogram_u64": (I, [P, P, U32, P, S]),
+    "tc_binning_pass_u32": (I, [P, P, P, P, P, U32, P, U32, S]),
+    "tc_binning_pass_u64": (I, [P, P, P, P, P, U32, P, U32, S]),
+    "tc_sort_keys_u32": (I, [P, P, P, U32, S]),
+    "tc_sort_pairs_u32": (I, [P, P, P, P, P, U32, S]),
+    "tc_sort_keys_u64": (I, [P, P, P, U32, S]),
+    "tc_sort_pairs_u64": (I, [P, P, P, P, P, U32, S]),
+    "tc_unique_u32": (I, [P, P, P, U32, P, S]),
+    "tc_unique_u64": (I, [P, P, P, U32, P, S]),
+    "tc_build_radix_tree_u32": (I, [P, P, U32, P, P, P, P, P, P, S]),
+    "tc_build_radix_tree_u64": (I, [P, P, U32, P, P, P, P, P, P, S]),
+    "tc_edge_count": (I, [P, P, P, P, U32, P, S]),
+    "tc_exclusive_scan_u32": (I, [P, P, P, U32, P, S]),
+    "tc_edge_count_scan": (I, [P, P, P, P, P, U32, P, S]),
+    "tc_build_octree_u32": (I, [P, P, U32, P, P, P, P, P, P, P, P, U32, P, F, F, I, S]),
+    "tc_build_octree_u64": (I, [P, P, U32, P, P, P, P, P, P, P, P, U32, P, F, F, I, S]),
+    "tc_octree_node_bound": (U64, [U64, I]),
+    "tc_tree_alloc": (I, [P, U32, I, C.POINTER(TcTree)]),
+    "tc_tree_free": (I, [P, C.POINTER(TcTree)]),
+    "tc_build_tree": (I, [P, P, U32, F, F, I, C.POINTER(TcTree), S]),
+    "tc_tree_read_counts": (I, [P, C.POINTER(TcTree), S]),
+    "tc_host_compute_morton": (I, [P, P, P, U32, F, F, U32]),
+    "tc_host_histogram": (I, [P, U32, P, P]),
+    "tc_host_radix_sort": (I, [P, P, U32, P, U32]),
+    "tc_host_unique": (I, [P, P, U32, C.POINTER(U32)]),
+    "tc_host_build_radix_tree": (I, [P, C.c_int32, P, P, P, P, P, P, U32]),
+    "tc_host_edge_count": (I, [P, P, P, P, U32, U32]),
+    "tc_host_partial_sum": (I, [P, P, U32, U32, P]),
+    "tc_host_build_octree": (I, [P, P, P, P, P, P, P, F, F, C.c_int32, P, P, P, U32]),
+    "tc_gen_uniform": (None, [P, U64, U64, F]),
+    "tc_gen_gaussian_mixture": (None, [P, U64, U64, U32, F, F]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libtc.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libtc.so not found at {LIB_PATH}: build it first (make -C {PKG_DIR}, or __graft_entry__.build())"
+            )
+        # libtc needs libamdhip64.so.7.  PyTorch ships its own copy with that
+        # soname; loading torch first makes libtc bind to the same runtime, so
+        # torch's streams and allocations are valid handles for libtc.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def extra_signatures(names_and_sigs: dict):
+    """Register signatures of further exported functions (used by submodules)."""
+    L = lib()
+    for name, (res, args) in names_and_sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+        SIGNATURES.setdefault(name, (res, args))
+
+
+def check(status: int, where: str = "") -> None:
+    if status != TC_OK:
+        raise TcError(status, where)

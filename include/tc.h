@@ -1,0 +1,264 @@
+/*
+ * tc.h -- C ABI of the MI355X-native tree-construction library (libtc.so).
+ *
+ * Stage chain (reference: ucsc-redwood/glsl-tree-construction, paths relative
+ * to vulkan/radixsort-vulkan/):
+ *
+ *   points --morton--> codes --sort--> sorted --unique--> keys
+ *          --radix tree--> (prefix_n, has_leaf_left/right, left_child, parent)
+ *          --edge count--> counts --exclusive scan--> offsets --octree--> nodes
+ *
+ * Conventions
+ *  - Device-pointer entry points (tc_<stage>) are asynchronous on the given HIP
+ *    stream (passed as void*; NULL is the legacy default stream).  The caller
+ *    owns every array it passes; the context owns scratch space.
+ *  - Host-array entry points (tc_host_<stage>) mirror the reference's per-stage
+ *    Rust functions one for one: they copy host arrays in, run the stage on the
+ *    GPU and copy results out, blocking.
+ *  - Every function returns TC_OK (0) or a negative tc_status; nothing throws.
+ *  - A tc_ctx is externally synchronised: use it from one host thread and one
+ *    stream at a time (its scratch is reused by every call).
+ *  - Point arrays are float[n][4] (x, y, z, w), 16 B per point.
+ *  - Counts are limited to 2^30 per sort (the lookback words pack a 30-bit count).
+ *  - Optional device counts: the tree stages take a host capacity and an
+ *    optional device pointer `d_n_keys` holding the number of unique keys U
+ *    (as written by tc_unique_*).  When it is non-NULL the stage reads U on the
+ *    device, so a whole build can be queued without a host synchronisation.
+ */
+#ifndef TC_H
+#define TC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tc_ctx tc_ctx;
+typedef void* tc_stream; /* hipStream_t */
+
+enum tc_status {
+    TC_OK = 0,
+    TC_ERR_INVALID = -1,     /* bad argument (NULL pointer, bad size, bad flag) */
+    TC_ERR_NOMEM = -2,       /* device or host allocation failed */
+    TC_ERR_TOO_LARGE = -3,   /* n above the 2^30 packing limit */
+    TC_ERR_HIP = -4,         /* a HIP runtime call failed */
+    TC_ERR_OVER_BUDGET = -5, /* request exceeds the memory budget */
+    TC_ERR_CAPACITY = -6,    /* output capacity too small */
+    TC_ERR_BUSY = -7,        /* queue full / try again */
+    TC_ERR_NOT_FOUND = -8,   /* unknown request id */
+    TC_ERR_NO_DEVICE = -9    /* no usable GPU */
+};
+
+#define TC_MAX_SORT_N (1u << 30)
+
+/* 64-byte octree node: OctNode of shader/octree.comp:14-20 (std430 stride 64 B)
+ * with an explicit pad so host and device strides agree. */
+typedef struct {
+    int32_t children[8];
+    float corner[4];
+    float cell_size;
+    int32_t child_node_mask;
+    int32_t child_leaf_mask;
+    int32_t pad_;
+} tc_oct_node;
+
+const char* tc_status_string(int status);
+const char* tc_version(void);
+
+/* Context: device, scratch (histograms, lookback words, tile counters). */
+int tc_create(tc_ctx** out, int device);
+int tc_destroy(tc_ctx* ctx);
+/* Pre-size scratch for sorts/scans of up to max_n elements so that later
+ * calls allocate nothing (needed before stream capture). */
+int tc_reserve(tc_ctx* ctx, uint32_t max_n);
+/* Launch parameters of the sort's binning pass: block size (256 or 512) and
+ * keys per thread (8, 12, 16 or 20).  0 keeps the current value. */
+int tc_set_sort_params(tc_ctx* ctx, int block, int keys_per_thread);
+int tc_get_sort_params(tc_ctx* ctx, int* block, int* keys_per_thread);
+
+/* Per-launch event profiler (the reference times each stage with
+ * Instant::now() around submit + fence, e.g. src/morton.rs:327-352).  When
+ * enabled, every kernel launch is bracketed by two HIP events on its own
+ * stream; collect synchronises on them and accumulates per-kernel totals.
+ * Kernel ids: 0 morton, 1 histogram, 2 binning, 3 unique, 4 scan,
+ * 5 radix_tree, 6 edge_count, 7 edge_count_scan, 8 octree_emit,
+ * 9 octree_link, 10 fill, 11 decode, 12 segmented. */
+int tc_profile_enable(tc_ctx* ctx, int on);
+int tc_profile_collect(tc_ctx* ctx);
+int tc_profile_reset(tc_ctx* ctx);
+int tc_profile_get(tc_ctx* ctx, int kernel_id, double* total_ms, uint64_t* launches);
+const char* tc_kernel_name(int kernel_id);
+
+/* ---- Morton encoding: k_ComputeMortonCode, shader/morton.comp:60-67;
+ *      host wrapper compute_morton, src/morton.rs:76-83. */
+int tc_morton30(tc_ctx* ctx, const float* pts_xyzw, uint32_t* codes, uint32_t n, float min_coord,
+                float range, tc_stream stream);
+/* 63-bit codes, 21 bits per axis (no reference counterpart; same formula with
+ * bit scale 2^21). */
+int tc_morton63(tc_ctx* ctx, const float* pts_xyzw, uint64_t* codes, uint32_t n, float min_coord,
+                float range, tc_stream stream);
+/* Decode codes to cell corners: morton32_to_xyz, src/morton.rs:43-59. */
+int tc_morton30_decode(tc_ctx* ctx, const uint32_t* codes, float* corners_xyzw, uint32_t n,
+                       float min_coord, float range, tc_stream stream);
+/* Host-only scalar decode (no GPU needed): src/morton.rs:43. */
+void tc_morton32_to_xyz(float ret[4], uint32_t code, float min_coord, float range);
+void tc_morton64_to_xyz(float ret[4], uint64_t code, float min_coord, float range);
+
+/* ---- Global histogram: k_GlobalHistogram, shader/histogram.comp:103-195;
+ *      host wrapper histogram(), src/histogram.rs:38.
+ * hist[256*d + b] = #{keys with digit d < b} (exclusive prefix, digit-major);
+ * 4 digits for u32 keys, 8 for u64. */
+int tc_global_histogram_u32(tc_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_t* hist_4x256,
+                            tc_stream stream);
+int tc_global_histogram_u64(tc_ctx* ctx, const uint64_t* keys, uint32_t n, uint32_t* hist_8x256,
+                            tc_stream stream);
+
+/* ---- One OneSweep digit-binning pass: k_DigitBinning,
+ *      shader/new_radix_sort.comp:116-314, Params{pass_num, radix_shift}
+ *      src/main.rs:175-180.  Stable scatter of src by digit
+ *      (key >> radix_shift) & 255 using hist (the full digit-major table from
+ *      tc_global_histogram_*).  Values may be NULL. */
+int tc_binning_pass_u32(tc_ctx* ctx, const uint32_t* src_k, uint32_t* dst_k, const uint32_t* src_v,
+                        uint32_t* dst_v, uint32_t n, const uint32_t* hist, uint32_t radix_shift,
+                        tc_stream stream);
+int tc_binning_pass_u64(tc_ctx* ctx, const uint64_t* src_k, uint64_t* dst_k, const uint32_t* src_v,
+                        uint32_t* dst_v, uint32_t n, const uint32_t* hist, uint32_t radix_shift,
+                        tc_stream stream);
+
+/* ---- Full stable LSD sorts (histogram + 4 or 8 binning passes), ascending.
+ * Contract of u_sort.sort(), src/main.rs:85; API shape of radix_sort,
+ * src/radixsort.rs:36.  The result ends in keys (and vals); alt buffers are
+ * ping-pong space of n elements. */
+int tc_sort_keys_u32(tc_ctx* ctx, uint32_t* keys, uint32_t* keys_alt, uint32_t n, tc_stream stream);
+int tc_sort_pairs_u32(tc_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                      uint32_t* vals_alt, uint32_t n, tc_stream stream);
+int tc_sort_keys_u64(tc_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, uint32_t n, tc_stream stream);
+int tc_sort_pairs_u64(tc_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                      uint32_t* vals_alt, uint32_t n, tc_stream stream);
+
+/* ---- Unique: k_CountUnique, shader/unique.comp:31-45; host wrapper
+ *      unique(), src/unique.rs:37.  out[0..U) = first key of each run of equal
+ *      keys; U is written to *d_n_unique (device).  out must not alias sorted. */
+int tc_unique_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out, uint32_t n,
+                  uint32_t* d_n_unique, tc_stream stream);
+int tc_unique_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out, uint32_t n,
+                  uint32_t* d_n_unique, tc_stream stream);
+
+/* ---- Karras binary radix tree: k_BuildRadixTree,
+ *      shader/build_radix_tree.comp:71-153; host wrapper build_radix_tree,
+ *      src/build_radix_tree.rs:34-43.  n_keys strictly increasing codes
+ *      (30-bit in u32, 63-bit in u64) give n_keys-1 internal nodes; each output
+ *      array holds n_keys-1 entries. */
+int tc_build_radix_tree_u32(tc_ctx* ctx, const uint32_t* codes, uint32_t n_keys,
+                            const uint32_t* d_n_keys, uint8_t* prefix_n, uint8_t* has_leaf_left,
+                            uint8_t* has_leaf_right, int32_t* left_child, int32_t* parent,
+                            tc_stream stream);
+int tc_build_radix_tree_u64(tc_ctx* ctx, const uint64_t* codes, uint32_t n_keys,
+                            const uint32_t* d_n_keys, uint8_t* prefix_n, uint8_t* has_leaf_left,
+                            uint8_t* has_leaf_right, int32_t* left_child, int32_t* parent,
+                            tc_stream stream);
+
+/* ---- Edge count: countEdges, shader/edge_count.comp:30-41; host wrapper
+ *      edge_count(), src/edge_count.rs:41-47.  n_brt = n_keys-1 entries;
+ *      entry 0 (the root) is 1. */
+int tc_edge_count(tc_ctx* ctx, const uint8_t* prefix_n, const int32_t* parent, uint32_t* edge_count,
+                  uint32_t n_brt, const uint32_t* d_n_keys, tc_stream stream);
+
+/* ---- Exclusive scan: k_PartialSum, shader/partial_sum.comp:23-39 with the
+ *      fix-up of src/main.rs:127-129; host wrapper partial_sum(),
+ *      src/partial_sum.rs:43.  out has n+1 entries; out[n] = total.
+ *      With d_n_keys the element count is (*d_n_keys - 1). */
+int tc_exclusive_scan_u32(tc_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                          const uint32_t* d_n_keys, tc_stream stream);
+/* Edge count and exclusive scan in one pass (writes both arrays). */
+int tc_edge_count_scan(tc_ctx* ctx, const uint8_t* prefix_n, const int32_t* parent,
+                       uint32_t* edge_count, uint32_t* offsets, uint32_t n_brt,
+                       const uint32_t* d_n_keys, tc_stream stream);
+
+/* ---- Octree: k_MakeOctNodes + k_LinkLeafNodes, shader/octree.comp:126-236,
+ *      root set-up src/main.rs:140-153; host wrapper build_octree(),
+ *      src/octree.rs:73-87.  nodes must hold offsets[n_brt] entries
+ *      (nodes_capacity is checked on the device; nodes beyond it are not
+ *      written).  Leaves are linked when link_leaves != 0. */
+int tc_build_octree_u32(tc_ctx* ctx, tc_oct_node* nodes, uint32_t nodes_capacity,
+                        const uint32_t* offsets, const uint32_t* edge_count, const uint32_t* codes,
+                        const uint8_t* prefix_n, const int32_t* parent, const int32_t* left_child,
+                        const uint8_t* has_leaf_left, const uint8_t* has_leaf_right, uint32_t n_brt,
+                        const uint32_t* d_n_keys, float min_coord, float range, int link_leaves,
+                        tc_stream stream);
+int tc_build_octree_u64(tc_ctx* ctx, tc_oct_node* nodes, uint32_t nodes_capacity,
+                        const uint32_t* offsets, const uint32_t* edge_count, const uint64_t* codes,
+                        const uint8_t* prefix_n, const int32_t* parent, const int32_t* left_child,
+                        const uint8_t* has_leaf_left, const uint8_t* has_leaf_right, uint32_t n_brt,
+                        const uint32_t* d_n_keys, float min_coord, float range, int link_leaves,
+                        tc_stream stream);
+/* Upper bound on the node count for n_keys unique codes (1 + sum over levels of
+ * min(8^L, n_keys/2)); the pipeline sizes its node array with it. */
+uint64_t tc_octree_node_bound(uint64_t n_keys, int code_bits);
+
+/* ---- Whole plain pipeline (the stage order of src/main.rs:60-170) on
+ *      device-resident buffers, queued without any host synchronisation. */
+typedef struct {
+    uint32_t capacity;        /* max points */
+    int code_bits;            /* 30 (u32 codes) or 63 (u64 codes) */
+    uint64_t node_capacity;   /* octree node capacity */
+    void* sorted;             /* [capacity] Morton codes, sorted in place */
+    void* alt;                /* [capacity] sort ping-pong space */
+    void* unique;             /* [capacity] unique codes */
+    uint8_t* prefix_n;        /* [capacity] radix tree */
+    uint8_t* has_leaf_left;
+    uint8_t* has_leaf_right;
+    int32_t* left_child;
+    int32_t* parent;
+    uint32_t* edge_count;     /* [capacity] */
+    uint32_t* offsets;        /* [capacity+1] */
+    tc_oct_node* nodes;       /* [node_capacity] */
+    uint32_t* d_counts;       /* device [4]: n_unique, n_brt, n_nodes, reserved */
+    /* filled by tc_tree_read_counts */
+    uint32_t n_points, n_unique, n_brt, n_nodes;
+} tc_tree;
+
+int tc_tree_alloc(tc_ctx* ctx, uint32_t capacity, int code_bits, tc_tree* tree);
+int tc_tree_free(tc_ctx* ctx, tc_tree* tree);
+int tc_build_tree(tc_ctx* ctx, const float* pts_xyzw, uint32_t n, float min_coord, float range,
+                  int link_leaves, tc_tree* tree, tc_stream stream);
+/* Synchronise the stream and copy the device counts into the tree struct. */
+int tc_tree_read_counts(tc_ctx* ctx, tc_tree* tree, tc_stream stream);
+
+/* ---- Host-array mirrors of the reference's per-stage functions (blocking).
+ * Argument order and meaning follow the Rust functions cited; sizes are
+ * explicit and results are written to caller arrays. */
+int tc_host_compute_morton(tc_ctx* ctx, const float* input_xyzw, uint32_t* morton_keys, uint32_t n,
+                           float min_coord, float range, uint32_t group_size);
+int tc_host_histogram(tc_ctx* ctx, uint32_t input_size, const uint32_t* input_data,
+                      uint32_t* global_hist);
+int tc_host_radix_sort(tc_ctx* ctx, const uint32_t* in_data, uint32_t n, uint32_t* out_data,
+                       uint32_t group_size);
+int tc_host_unique(tc_ctx* ctx, uint32_t* u_sort, uint32_t n, uint32_t* n_unique);
+int tc_host_build_radix_tree(tc_ctx* ctx, int32_t n, const uint32_t* codes, uint8_t* prefix_n,
+                             uint8_t* has_leaf_left, uint8_t* has_leaf_right, int32_t* left_child,
+                             int32_t* parent, uint32_t group_size);
+int tc_host_edge_count(tc_ctx* ctx, const uint8_t* prefix_n, const int32_t* parent,
+                       uint32_t* u_edge_count, uint32_t n_brt_nodes, uint32_t group_size);
+int tc_host_partial_sum(tc_ctx* ctx, const uint32_t* u_edge_count, uint32_t first, uint32_t last,
+                        uint32_t* d_first);
+int tc_host_build_octree(tc_ctx* ctx, tc_oct_node* oct_nodes, const uint32_t* node_offsets,
+                         const uint32_t* rt_node_counts, const uint32_t* codes,
+                         const uint8_t* rt_prefix_n, const int32_t* rt_parents, float min_coord,
+                         float range, int32_t n, const uint8_t* rt_has_leaf_left,
+                         const uint8_t* rt_has_leaf_right, const int32_t* rt_left_child,
+                         uint32_t group_size);
+
+/* ---- Seeded input generators (host only, multi-threaded, no GPU needed).
+ * Counter-based SplitMix64 streams: the value for point i depends only on
+ * (seed, i), so any thread count gives identical arrays. */
+void tc_gen_uniform(float* pts_xyzw, uint64_t n, uint64_t seed, float extent);
+void tc_gen_gaussian_mixture(float* pts_xyzw, uint64_t n, uint64_t seed, uint32_t n_centres,
+                             float sigma, float extent);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

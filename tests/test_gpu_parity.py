@@ -1,0 +1,254 @@
+"""GPU parity: every stage's HIP output is bit-exact against the CPU oracle on
+the same seeded inputs, through the C ABI.  Sizes stay where the oracle
+finishes in seconds; full-size runs are covered by property checks in
+test_gpu_scale.py."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from gpu_util import dev, empty, host, run_stages
+
+pytestmark = pytest.mark.gpu
+
+STAGE_KEYS = ("codes", "hist", "sorted", "unique", "prefix_n", "has_leaf_left", "has_leaf_right", "left_child",
+              "parent", "edge_count", "offsets")
+
+
+def oracle_all(oracle, pts, mn, rg, bits):
+    r = oracle.build_tree(pts, mn, rg, code_bits=bits)
+    r["hist"] = oracle.histogram(r["codes"])
+    r["nodes"] = r["nodes"].view(np.uint8).reshape(-1)
+    return r
+
+
+def assert_same(got, want, keys=STAGE_KEYS + ("nodes",)):
+    for k in keys:
+        g, w = np.asarray(got[k]), np.asarray(want[k])
+        assert g.shape == w.shape, (k, g.shape, w.shape)
+        if not np.array_equal(g, w):
+            bad = np.flatnonzero(g != w)
+            raise AssertionError(f"{k}: {len(bad)} mismatches, first at {bad[:5]}: got {g[bad[:5]]} want {w[bad[:5]]}")
+
+
+@pytest.mark.parametrize("name", ["uniform_2048_seed11.npz", "gauss_4096_seed12.npz", "uniform63_1024_seed13.npz"])
+def test_golden_fixture_stage_by_stage(gpu_ctx, name):
+    g = np.load(os.path.join(GOLDEN, name))
+    got = run_stages(gpu_ctx, g["points"], float(g["min_coord"]), float(g["range"]), int(g["code_bits"]))
+    assert_same(got, {k: g[k] for k in STAGE_KEYS + ("nodes",)})
+
+
+def test_golden_pairs_u64(gpu_ctx):
+    g = np.load(os.path.join(GOLDEN, "pairs_u64_3000_seed14.npz"))
+    n = len(g["keys"])
+    k, v = dev(g["keys"]), dev(g["vals"])
+    ka, va = empty(n, np.uint64), empty(n, np.uint32)
+    gpu_ctx.sort(k, ka, v, va)
+    assert np.array_equal(host(k, np.uint64), g["sorted_keys"])
+    assert np.array_equal(host(v, np.uint32), g["sorted_vals"])
+    h = empty(2048, np.uint32)
+    gpu_ctx.global_histogram(dev(g["keys"]), h)
+    assert np.array_equal(host(h, np.uint32), g["hist"])
+
+
+def test_config1_uniform_2p20(gpu_ctx, oracle, tclib):
+    """Plain pipeline, single cloud: 2^20 points uniform in [0,1024)^3, every
+    array bit-exact (each stage through its own entry point)."""
+    pts = tclib.gen_uniform(1 << 20, 0xC0FFEE01)
+    got = run_stages(gpu_ctx, pts, 0.0, 1024.0, 30)
+    want = oracle_all(oracle, pts, 0.0, 1024.0, 30)
+    assert_same(got, want)
+
+
+def test_gaussian_mixture_heavy_duplicates(gpu_ctx, oracle, tclib):
+    pts = tclib.gen_gaussian_mixture(1 << 19, 0xC0FFEE02, 64, 8.0)
+    got = run_stages(gpu_ctx, pts, 0.0, 1024.0, 30)
+    want = oracle_all(oracle, pts, 0.0, 1024.0, 30)
+    assert len(want["unique"]) < len(pts)
+    assert_same(got, want)
+
+
+def test_63bit_tree(gpu_ctx, oracle, tclib):
+    pts = tclib.gen_uniform(1 << 16, 77, 2097152.0)
+    got = run_stages(gpu_ctx, pts, 0.0, 2097152.0, 63)
+    want = oracle_all(oracle, pts, 0.0, 2097152.0, 63)
+    assert_same(got, want)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 4095, 4096, 4097, 7679, 7680, 7681, 12345])
+def test_edge_sizes(gpu_ctx, oracle, tclib, n):
+    pts = tclib.gen_uniform(n, 1000 + n)
+    got = run_stages(gpu_ctx, pts, 0.0, 1024.0, 30)
+    assert_same(got, oracle_all(oracle, pts, 0.0, 1024.0, 30))
+
+
+def test_degenerate_distributions(gpu_ctx, oracle):
+    cases = {}
+    cases["all_same_point"] = np.tile(np.array([[3.5, 700.25, 12.0, 1.0]], np.float32), (5000, 1))
+    p = np.zeros((4000, 4), np.float32)
+    p[:, 0] = np.arange(4000) % 2  # codes 0 and 1 only (differ in bit 0)
+    p[:, 3] = 1
+    cases["bit0_only"] = p
+    p = np.zeros((3000, 4), np.float32)
+    p[:, :3] = np.random.default_rng(5).integers(0, 4, size=(3000, 3))  # one occupied corner cell block
+    p[:, 3] = 1
+    cases["tiny_cell_block"] = p
+    p = np.zeros((2, 4), np.float32)
+    p[1, :3] = 1023.5
+    cases["two_extremes"] = p
+    for name, pts in cases.items():
+        got = run_stages(gpu_ctx, pts, 0.0, 1024.0, 30)
+        assert_same(got, oracle_all(oracle, pts, 0.0, 1024.0, 30))
+
+
+def test_out_of_range_inputs_match_oracle(gpu_ctx, oracle):
+    """NaN, negative and >= range coordinates quantise by the defined rule on both sides."""
+    pts = np.array([[np.nan, -1.0, 1024.0, 1.0], [1e30, -1e30, 0.0, 1.0], [-0.0, 1023.999, 2047.0, 1.0],
+                    [5.0, np.inf, -np.inf, 1.0]], np.float32)
+    codes = empty(4, np.uint32)
+    gpu_ctx.morton30(dev(pts), codes, 0.0, 1024.0)
+    assert np.array_equal(host(codes, np.uint32), oracle.compute_morton(pts, 0.0, 1024.0))
+    c63 = empty(4, np.uint64)
+    gpu_ctx.morton63(dev(pts), c63, 0.0, 1024.0)
+    assert np.array_equal(host(c63, np.uint64), oracle.compute_morton63(pts, 0.0, 1024.0))
+
+
+def test_nonunit_min_range(gpu_ctx, oracle, tclib):
+    pts = tclib.gen_uniform(50_000, 31, 1.0)
+    pts[:, :3] = pts[:, :3] * np.float32(37.0) - np.float32(11.0)
+    got = run_stages(gpu_ctx, pts, -11.0, 37.0, 30)
+    assert_same(got, oracle_all(oracle, pts, -11.0, 37.0, 30))
+
+
+def test_decode_kernel(gpu_ctx, oracle):
+    codes = np.random.default_rng(6).integers(0, 1 << 30, size=10_000, dtype=np.uint64).astype(np.uint32)
+    out = empty(4 * len(codes), np.float32)
+    gpu_ctx.morton30_decode(dev(codes), out, -3.0, 17.0)
+    got = host(out, np.float32).reshape(-1, 4)
+    want = np.stack([oracle.morton32_to_xyz(int(c), -3.0, 17.0) for c in codes])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("block,kpt", [(256, 8), (256, 12), (256, 16), (256, 20), (512, 8), (512, 12),
+                                       (512, 16), (512, 20)])
+def test_sort_launch_shapes(gpu_ctx, oracle, block, kpt):
+    """Every binning launch shape the tuner may pick gives the stable order."""
+    r = np.random.default_rng(block * 100 + kpt)
+    n = 300_001
+    keys = r.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    keys[::7] = keys[3]
+    vals = np.arange(n, dtype=np.uint32)
+    prev = gpu_ctx.sort_params()
+    gpu_ctx.set_sort_params(block, kpt)
+    try:
+        k, v = dev(keys), dev(vals)
+        gpu_ctx.sort(k, empty(n, np.uint32), v, empty(n, np.uint32))
+        sk, sv = oracle.radix_sort(keys, vals)
+        assert np.array_equal(host(k, np.uint32), sk) and np.array_equal(host(v, np.uint32), sv)
+        k64 = (keys.astype(np.uint64) << np.uint64(31)) | r.integers(0, 1 << 31, size=n, dtype=np.uint64)
+        k, v = dev(k64), dev(vals)
+        gpu_ctx.sort(k, empty(n, np.uint64), v, empty(n, np.uint32))
+        sk, sv = oracle.radix_sort(k64, vals)
+        assert np.array_equal(host(k, np.uint64), sk) and np.array_equal(host(v, np.uint32), sv)
+    finally:
+        gpu_ctx.set_sort_params(*prev)
+
+
+def test_binning_pass_api(gpu_ctx, oracle):
+    r = np.random.default_rng(9)
+    keys = r.integers(0, 1 << 32, size=100_000, dtype=np.uint64).astype(np.uint32)
+    hist = oracle.histogram(keys)
+    for shift in (0, 8, 16, 24):
+        dst = empty(len(keys), np.uint32)
+        gpu_ctx.binning_pass(dev(keys), dst, None, None, dev(hist), shift)
+        want, _ = oracle.binning_pass(keys, None, hist[32 * shift:32 * shift + 256], shift)
+        assert np.array_equal(host(dst, np.uint32), want)
+
+
+def test_repeat_runs_identical(gpu_ctx, tclib):
+    """Same input K times -> identical outputs (catches look-back and race bugs)."""
+    pts = tclib.gen_gaussian_mixture(1 << 18, 4242, 16, 4.0)
+    first = run_stages(gpu_ctx, pts, 0.0, 1024.0, 30)
+    for _ in range(4):
+        again = run_stages(gpu_ctx, pts, 0.0, 1024.0, 30)
+        assert_same(again, first)
+
+
+def test_fused_pipeline_matches_oracle(gpu_ctx, oracle, tclib):
+    """tc_build_tree (fused edge-count+scan, device-side counts, no host sync)."""
+    import tc
+
+    for n, seed, gauss in ((1 << 20, 0xC0FFEE01, False), (300_000, 55, True), (5, 1, False), (1, 2, False)):
+        pts = tclib.gen_gaussian_mixture(n, seed) if gauss else tclib.gen_uniform(n, seed)
+        tree = tc.Tree(gpu_ctx, n, 30)
+        tree.build(dev(pts), n, 0.0, 1024.0)
+        got = tree.download()
+        want = oracle_all(oracle, pts, 0.0, 1024.0, 30)
+        assert got["n_unique"] == len(want["unique"])
+        assert_same(got, want, ("sorted", "unique", "prefix_n", "has_leaf_left", "has_leaf_right", "left_child",
+                                "parent", "edge_count", "offsets"))
+        assert np.array_equal(got["nodes"].view(np.uint8).reshape(-1), want["nodes"])
+        tree.free()
+
+
+def test_fused_pipeline_63bit(gpu_ctx, oracle, tclib):
+    import tc
+
+    pts = tclib.gen_uniform(100_000, 63, 2097152.0)
+    tree = tc.Tree(gpu_ctx, len(pts), 63)
+    tree.build(dev(pts), len(pts), 0.0, 2097152.0)
+    got = tree.download()
+    want = oracle_all(oracle, pts, 0.0, 2097152.0, 63)
+    assert_same(got, want, ("sorted", "unique", "prefix_n", "left_child", "parent", "edge_count", "offsets"))
+    assert np.array_equal(got["nodes"].view(np.uint8).reshape(-1), want["nodes"])
+    tree.free()
+
+
+def test_host_mirror_functions(gpu_ctx, oracle, tclib):
+    """The reference-shaped host functions (src/main.rs:60-170 call sequence)."""
+    import tc
+
+    n = 100_000
+    pts = tclib.gen_uniform(n, 99)
+    codes = tc.compute_morton(pts, n, 0.0, 1024.0, 256, ctx=gpu_ctx)
+    assert np.array_equal(codes, oracle.compute_morton(pts, 0.0, 1024.0))
+    assert np.array_equal(tc.histogram(n, codes, ctx=gpu_ctx), oracle.histogram(codes))
+    s = tc.radix_sort(codes, n, 256, ctx=gpu_ctx)
+    assert np.array_equal(s, np.sort(codes))
+    u_sort = s.copy()
+    nu = tc.unique(u_sort, n, ctx=gpu_ctx)
+    uniq = oracle.unique(s)
+    assert nu == len(uniq) and np.array_equal(u_sort[:nu], uniq)
+    nb = nu - 1
+    pn, hll, hlr = np.zeros(nb, np.uint8), np.zeros(nb, np.uint8), np.zeros(nb, np.uint8)
+    left, par = np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+    tc.build_radix_tree(nb, u_sort[:nu], pn, hll, hlr, left, par, 256, ctx=gpu_ctx)
+    t = oracle.build_radix_tree(uniq)
+    for k, v in dict(prefix_n=pn, has_leaf_left=hll, has_leaf_right=hlr, left_child=left, parent=par).items():
+        assert np.array_equal(v, t[k]), k
+    ec = np.zeros(nb, np.uint32)
+    tc.edge_count(pn, par, ec, nb, 256, ctx=gpu_ctx)
+    assert np.array_equal(ec, oracle.edge_count(pn, par))
+    offs = np.zeros(nu, np.uint32)
+    tc.partial_sum(ec, 0, nb, offs, ctx=gpu_ctx)
+    assert np.array_equal(offs, oracle.exclusive_scan(ec))
+    nodes = np.zeros(int(offs[nb]), tc.OCT_NODE_DTYPE)
+    tc.build_octree(nodes, offs, ec, u_sort[:nu], pn, par, 0.0, 1024.0, nb, hll, hlr, left, 256, ctx=gpu_ctx)
+    want = oracle.build_octree(offs, ec, uniq, t, 0.0, 1024.0)
+    assert np.array_equal(nodes.view(np.uint8), want.view(np.uint8))
+
+
+def test_invalid_arguments_rejected(gpu_ctx):
+    import ctypes
+
+    import tc
+
+    L = tc.lib()
+    h = gpu_ctx.handle
+    assert L.tc_sort_keys_u32(h, None, None, 10, None) == tc._lib.TC_ERR_INVALID
+    assert L.tc_sort_keys_u32(h, None, None, (1 << 30) + 1, None) in (tc._lib.TC_ERR_INVALID, tc._lib.TC_ERR_TOO_LARGE)
+    assert L.tc_set_sort_params(h, 128, 16) == tc._lib.TC_ERR_INVALID
+    assert L.tc_morton30(h, ctypes.c_void_p(8), ctypes.c_void_p(16), 4, 0.0, 1.0, None) == tc._lib.TC_ERR_INVALID
+    k = empty(16, np.uint32)
+    assert L.tc_sort_keys_u32(h, k.data_ptr(), k.data_ptr(), (1 << 30) + 1, None) == tc._lib.TC_ERR_TOO_LARGE
