@@ -1,0 +1,554 @@
+// batch.hip -- many clouds of uneven sizes built in one persistent, segmented
+// launch per stage, optionally captured as a HIP graph.
+//
+// No reference counterpart: the reference builds one cloud per call and creates
+// a Vulkan device per stage (src/main.rs:60-170).  Each cloud's outputs here
+// are bit-identical to a plain single-cloud build (tc_build_tree) of the same
+// points.
+//
+// Layout: clouds are concatenated.  Cloud c owns points [pt_off[c],
+// pt_off[c+1]); its sort tiles [first_tile[c], first_tile[c+1]) never cross
+// into another cloud; its unique codes and radix-node slots are
+// [ubase[c], ubase[c+1]) (U_c keys, U_c - 1 nodes, the last slot unused); its
+// octree nodes are [node_base[c], node_base[c+1]).  Stored tree links are
+// cloud-local indices.
+//
+// Every kernel reads the batch shape from device memory and uses a capacity
+// grid (persistent loops), so one captured graph replays for any batch that
+// fits the capacity: the host only uploads the new plan before the replay.
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "sort_core.hpp"
+#include "tree_core.hpp"
+
+struct tc_batch {
+    tc_ctx* ctx = nullptr;
+    uint32_t cap_clouds = 0, cap_points = 0, cap_tiles = 0;
+    int code_bits = 30;
+    int block = 256, kpt = 16;
+    uint32_t tile = 4096;
+    // Device plan block (one upload per batch): [desc 8][pt_off C+1][first_tile C+1][min C][range C].
+    uint32_t* plan = nullptr;
+    size_t plan_words = 0;
+    uint32_t* h_plan = nullptr;  // pinned staging
+    hipEvent_t plan_copied = nullptr;
+    // Device work arrays.
+    uint32_t* seg_hist = nullptr;  // [cap_clouds * digits * 256]
+    uint32_t* lb = nullptr;        // 2 x cap_tiles x 256
+    uint32_t* ctrs = nullptr;      // [16] per-pass tile counters
+    uint32_t* ubase = nullptr;     // [cap_clouds + 1]
+    uint32_t* node_base = nullptr; // [cap_clouds + 1]
+    uint32_t* totals = nullptr;    // [4]: unique total, node total
+    // Outputs (concatenated).
+    void* codes = nullptr;
+    void* alt = nullptr;
+    void* uniq = nullptr;
+    uint8_t* prefix_n = nullptr;
+    uint8_t* hll = nullptr;
+    uint8_t* hlr = nullptr;
+    int32_t* left = nullptr;
+    int32_t* parent = nullptr;
+    uint32_t* ec = nullptr;
+    uint32_t* offsets = nullptr;  // [cap_points + 1]
+    tc_oct_node* nodes = nullptr;
+    uint64_t node_cap = 0;
+    // Host copy of the current plan shape.
+    uint32_t n_clouds = 0, n_points = 0, n_tiles = 0;
+    // Graph.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const float* graph_pts = nullptr;
+    int graph_link = -1;
+};
+
+namespace {
+
+constexpr int SEG_BLOCK = 256;
+
+// Plan views.
+struct Plan {
+    const uint32_t* desc;  // [0] n_clouds [1] n_tiles [2] n_points
+    const uint32_t* pt_off;
+    const uint32_t* first_tile;
+    const float* min_c;
+    const float* range_c;
+};
+
+Plan plan_view(const tc_batch* b) {
+    Plan p;
+    const uint32_t C = b->cap_clouds;
+    p.desc = b->plan;
+    p.pt_off = b->plan + 8;
+    p.first_tile = p.pt_off + (C + 1);
+    p.min_c = reinterpret_cast<const float*>(p.first_tile + (C + 1));
+    p.range_c = p.min_c + C;
+    return p;
+}
+
+// Largest c in [0, n-1] with off[c] <= x (the non-empty segment holding x).
+TC_DEV uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Morton codes + per-cloud digit histograms, one sort tile at a time (the
+// histogram is fused into the producer, so the codes are read only by the
+// binning passes).  Blocks take contiguous runs of tiles and flush their LDS
+// histogram to the cloud's counters when the cloud changes.
+template <typename K>
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const float4* __restrict__ pts,
+                                                               K* __restrict__ codes, uint32_t* __restrict__ seg_hist,
+                                                               uint32_t tile_size) {
+    constexpr int DIG = (int)sizeof(K);
+    __shared__ uint32_t s_h[DIG * 256];
+    __shared__ uint32_t s_cloud;
+    const uint32_t C = p.desc[0], T = p.desc[1];
+    const uint32_t chunk = ceil_div_u32(T, gridDim.x);
+    const uint32_t t0 = blockIdx.x * chunk;
+    const uint32_t t1 = t0 + chunk < T ? t0 + chunk : T;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < DIG * 256; i += SEG_BLOCK) s_h[i] = 0;
+    uint32_t cur = 0xFFFFFFFFu;
+    for (uint32_t t = t0; t < t1; ++t) {
+        if (tid == 0) s_cloud = seg_find(p.first_tile, C, t);
+        __syncthreads();
+        const uint32_t c = s_cloud;
+        if (c != cur) {
+            if (cur != 0xFFFFFFFFu) {
+                for (uint32_t i = tid; i < DIG * 256; i += SEG_BLOCK) {
+                    const uint32_t v = s_h[i];
+                    if (v) atomicAdd(&seg_hist[(size_t)cur * DIG * 256 + i], v);
+                    s_h[i] = 0;
+                }
+                __syncthreads();
+            }
+            cur = c;
+        }
+        const uint32_t start = p.pt_off[c] + (t - p.first_tile[c]) * tile_size;
+        const uint32_t end_c = p.pt_off[c + 1];
+        const uint32_t end = start + tile_size < end_c ? start + tile_size : end_c;
+        const float mn = p.min_c[c], rg = p.range_c[c];
+        for (uint32_t i = start + tid; i < end; i += SEG_BLOCK) {
+            const float4 q = pts[i];
+            K code;
+            if constexpr (sizeof(K) == 4) {
+                code = d_split3_10(d_quantize(q.x, mn, rg, 1024.0f)) | (d_split3_10(d_quantize(q.y, mn, rg, 1024.0f)) << 1) |
+                       (d_split3_10(d_quantize(q.z, mn, rg, 1024.0f)) << 2);
+            } else {
+                code = d_split3_21(d_quantize(q.x, mn, rg, 2097152.0f)) |
+                       (d_split3_21(d_quantize(q.y, mn, rg, 2097152.0f)) << 1) |
+                       (d_split3_21(d_quantize(q.z, mn, rg, 2097152.0f)) << 2);
+            }
+            codes[i] = code;
+#pragma unroll
+            for (int d = 0; d < DIG; ++d) atomicAdd(&s_h[d * 256 + (uint32_t)((code >> (8 * d)) & 255u)], 1u);
+        }
+        __syncthreads();
+    }
+    if (cur != 0xFFFFFFFFu) {
+        for (uint32_t i = tid; i < DIG * 256; i += SEG_BLOCK) {
+            const uint32_t v = s_h[i];
+            if (v) atomicAdd(&seg_hist[(size_t)cur * DIG * 256 + i], v);
+        }
+    }
+}
+
+// Per cloud and digit: counts -> exclusive prefixes, in place.
+template <int DIG>
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_prefix(Plan p, uint32_t* __restrict__ seg_hist) {
+    __shared__ uint32_t s_tmp[SEG_BLOCK / 64 + 2];
+    const uint32_t C = p.desc[0];
+    for (uint32_t c = blockIdx.x; c < C; c += gridDim.x) {
+        for (int d = 0; d < DIG; ++d) {
+            uint32_t* h = seg_hist + ((size_t)c * DIG + d) * 256;
+            const uint32_t v = h[threadIdx.x];
+            uint32_t tot;
+            const uint32_t ex = block_exclusive_scan<SEG_BLOCK>(v, s_tmp, &tot);
+            h[threadIdx.x] = ex;
+        }
+    }
+}
+
+// One OneSweep pass over every cloud: tiles claimed in order, look-back
+// confined to the cloud's own tiles.
+template <typename K, int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_seg_onesweep(Plan p, const K* __restrict__ sk, K* __restrict__ dk,
+                                                        uint32_t shift, uint32_t pass, const uint32_t* __restrict__ seg_hist,
+                                                        uint32_t* __restrict__ lb_cur, uint32_t* __restrict__ lb_next,
+                                                        uint32_t* __restrict__ tile_ctr) {
+    using Sh = OnesweepShared<K, BLOCK, KPT>;
+    constexpr int DIG = (int)sizeof(K);
+    __shared__ Sh sm;
+    const uint32_t C = p.desc[0], T = p.desc[1];
+    while (true) {
+        if (threadIdx.x == 0) {
+            const uint32_t t = atomicAdd(tile_ctr, 1u);
+            sm.tile = t;
+            sm.aux = t < T ? seg_find(p.first_tile, C, t) : 0u;
+        }
+        __syncthreads();
+        const uint32_t tile = sm.tile;
+        if (tile >= T) break;
+        const uint32_t c = sm.aux;
+        const uint32_t first = p.first_tile[c];
+        const uint32_t base = p.pt_off[c] + (tile - first) * Sh::TILE;
+        const uint32_t end = p.pt_off[c + 1];
+        const uint32_t cnt = end - base < Sh::TILE ? end - base : Sh::TILE;
+        onesweep_tile<K, false, BLOCK, KPT>(sm, sk, dk, nullptr, nullptr, base, cnt, shift, tile, first,
+                                            seg_hist + ((size_t)c * DIG + pass) * 256, p.pt_off[c], lb_cur, lb_next);
+        __syncthreads();
+    }
+}
+
+// Fix-ups after a segmented scan: empty segments get the base of the next
+// non-empty one; entry C gets the total.
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_finalize(const uint32_t* __restrict__ desc,
+                                                            const uint32_t* __restrict__ seg_off,
+                                                            uint32_t* __restrict__ base_out,
+                                                            const uint32_t* __restrict__ total) {
+    const uint32_t C = desc[0];
+    const uint32_t tot = *total;
+    for (uint32_t c = blockIdx.x * SEG_BLOCK + threadIdx.x; c <= C; c += gridDim.x * SEG_BLOCK) {
+        if (c == C) {
+            base_out[C] = tot;
+            continue;
+        }
+        if (seg_off[c] != seg_off[c + 1]) continue;  // written by the scan
+        uint32_t k = c + 1;
+        while (k < C && seg_off[k] == seg_off[k + 1]) ++k;
+        // k is the next non-empty segment, or C.  Its base is the exclusive
+        // prefix at its first element; read it from the scan's output array
+        // position instead of base_out (which the scan writes concurrently
+        // only for non-empty segments, already finished here).
+        base_out[c] = (k == C) ? tot : base_out[k];
+    }
+}
+
+// Unique-key slots: radix tree (and edge counts) per cloud.  The last slot of
+// each cloud has no radix node; its edge count is 0 so the node scan skips it.
+template <typename K>
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(const uint32_t* __restrict__ desc, const uint32_t* __restrict__ ubase,
+                                                       const K* __restrict__ uniq, uint8_t* __restrict__ prefix_n,
+                                                       uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr,
+                                                       int32_t* __restrict__ left, int32_t* __restrict__ parent,
+                                                       uint32_t* __restrict__ ec) {
+    const uint32_t C = desc[0];
+    const uint32_t total = ubase[C];
+    const uint32_t stride = gridDim.x * SEG_BLOCK;
+    for (uint32_t g = blockIdx.x * SEG_BLOCK + threadIdx.x; g < total; g += stride) {
+        const uint32_t c = seg_find(ubase, C, g);
+        const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
+        const uint32_t i = g - b;
+        if (i + 1 < nk) brt_node<K>(uniq + b, nk, i, prefix_n + b, hll + b, hlr + b, left + b, parent + b, ec + b);
+        else ec[g] = 0u;
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __restrict__ desc, const uint32_t* __restrict__ ubase,
+                                                          const uint32_t* __restrict__ node_base, tc_oct_node* __restrict__ nodes,
+                                                          uint32_t node_cap, const uint32_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ ec, const K* __restrict__ uniq,
+                                                          const uint8_t* __restrict__ prefix_n, const int32_t* __restrict__ left,
+                                                          const uint8_t* __restrict__ hll, const uint8_t* __restrict__ hlr,
+                                                          int link_leaves, Plan p) {
+    const uint32_t C = desc[0];
+    const uint32_t total = ubase[C];
+    const uint32_t stride = gridDim.x * SEG_BLOCK;
+    for (uint32_t g = blockIdx.x * SEG_BLOCK + threadIdx.x; g < total; g += stride) {
+        const uint32_t c = seg_find(ubase, C, g);
+        const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
+        const uint32_t i = g - b;
+        if (i + 1 >= nk) continue;
+        octree_build_node<K>(nodes, node_cap, node_base[c], offsets + b, ec + b, uniq + b, prefix_n + b, left + b,
+                             hll + b, hlr + b, i, nk - 1, prefix_n[b] / 3, link_leaves, p.min_c[c], p.range_c[c]);
+    }
+}
+
+template <typename K, int BLOCK, int KPT>
+void launch_seg_pass(tc_batch* b, hipStream_t s, uint32_t grid, const K* src, K* dst, uint32_t pass) {
+    const Plan p = plan_view(b);
+    uint32_t* lb0 = b->lb + (size_t)(pass & 1) * b->cap_tiles * 256;
+    uint32_t* lb1 = b->lb + (size_t)((pass + 1) & 1) * b->cap_tiles * 256;
+    hipLaunchKernelGGL((k_seg_onesweep<K, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, p, src, dst, 8u * pass, pass,
+                       b->seg_hist, lb0, (pass + 1 < sizeof(K)) ? lb1 : nullptr, b->ctrs + pass);
+}
+
+template <typename K>
+int seg_pass(tc_batch* b, hipStream_t s, const K* src, K* dst, uint32_t pass) {
+    // Persistent grid: enough workgroups to fill the GPU, never more than tiles.
+    uint32_t grid = (uint32_t)b->ctx->num_cus * (b->block == 256 ? 4 : 2);
+    if (grid > b->cap_tiles) grid = b->cap_tiles;
+    if (grid == 0) grid = 1;
+#define TC_SEG_CASE(B, KP)                                            \
+    if (b->block == B && b->kpt == KP) {                              \
+        launch_seg_pass<K, B, KP>(b, s, grid, src, dst, pass);        \
+        TC_LAUNCH_CHECK();                                            \
+        return TC_OK;                                                 \
+    }
+    TC_SEG_CASE(256, 8)
+    TC_SEG_CASE(256, 12)
+    TC_SEG_CASE(256, 16)
+    TC_SEG_CASE(256, 20)
+    TC_SEG_CASE(512, 8)
+    TC_SEG_CASE(512, 12)
+    TC_SEG_CASE(512, 16)
+    TC_SEG_CASE(512, 20)
+#undef TC_SEG_CASE
+    return TC_ERR_INVALID;
+}
+
+template <typename K>
+int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s) {
+    tc_ctx* ctx = b->ctx;
+    constexpr int DIG = (int)sizeof(K);
+    const Plan p = plan_view(b);
+    const uint32_t C = b->cap_clouds;
+    K* codes = static_cast<K*>(b->codes);
+    K* alt = static_cast<K*>(b->alt);
+    K* uniq = static_cast<K*>(b->uniq);
+    ProfScope prof(ctx, s, K_SEG);
+    // Clear per-cloud histogram counters, tile counters and look-back region 0.
+    TC_TRY(tc_fill_u32(ctx, b->seg_hist, 0u, (size_t)C * DIG * 256, s));
+    TC_TRY(tc_fill_u32(ctx, b->ctrs, 0u, 16, s));
+    TC_TRY(tc_fill_u32(ctx, b->lb, 0u, (size_t)b->cap_tiles * 256, s));
+    uint32_t grid = (uint32_t)ctx->num_cus * 4;
+    if (grid > b->cap_tiles) grid = b->cap_tiles;
+    hipLaunchKernelGGL(k_seg_morton_hist<K>, dim3(grid), dim3(SEG_BLOCK), 0, s, p,
+                       reinterpret_cast<const float4*>(pts), codes, b->seg_hist, b->tile);
+    TC_LAUNCH_CHECK();
+    uint32_t cgrid = C < (uint32_t)ctx->num_cus * 4 ? C : (uint32_t)ctx->num_cus * 4;
+    hipLaunchKernelGGL(k_seg_prefix<DIG>, dim3(cgrid ? cgrid : 1), dim3(SEG_BLOCK), 0, s, p, b->seg_hist);
+    TC_LAUNCH_CHECK();
+    K* src = codes;
+    K* dst = alt;
+    for (int pass = 0; pass < DIG; ++pass) {
+        TC_TRY(seg_pass<K>(b, s, src, dst, (uint32_t)pass));
+        K* t = src; src = dst; dst = t;
+    }
+    // Even pass count: sorted codes are back in `codes`.
+    TC_TRY(tc_launch_scan(ctx, DIG == 4 ? MODE_SEG_UNIQUE32 : MODE_SEG_UNIQUE64, codes, nullptr, uniq, nullptr,
+                          b->totals, b->cap_points, nullptr, p.pt_off, C, p.desc, b->ubase, s));
+    const uint32_t fgrid = ceil_div_u32(C + 1, SEG_BLOCK);
+    hipLaunchKernelGGL(k_seg_finalize, dim3(fgrid), dim3(SEG_BLOCK), 0, s, p.desc, p.pt_off, b->ubase, b->totals);
+    TC_LAUNCH_CHECK();
+    const uint32_t tgrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK);
+    hipLaunchKernelGGL(k_seg_brt<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, uniq, b->prefix_n,
+                       b->hll, b->hlr, b->left, b->parent, b->ec);
+    TC_LAUNCH_CHECK();
+    TC_TRY(tc_launch_scan(ctx, MODE_SEG_SCAN, b->ec, nullptr, b->offsets, nullptr, b->totals + 1, b->cap_points,
+                          nullptr, b->ubase, C, p.desc, b->node_base, s));
+    hipLaunchKernelGGL(k_seg_finalize, dim3(fgrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, b->node_base,
+                       b->totals + 1);
+    TC_LAUNCH_CHECK();
+    const uint32_t ncap = (uint32_t)(b->node_cap < 0xFFFFFFFFull ? b->node_cap : 0xFFFFFFFFull);
+    hipLaunchKernelGGL(k_seg_octree<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, b->node_base,
+                       b->nodes, ncap, b->offsets, b->ec, uniq, b->prefix_n, b->left, b->hll, b->hlr, link_leaves, p);
+    TC_LAUNCH_CHECK();
+    return TC_OK;
+}
+
+int run_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s) {
+    return b->code_bits == 30 ? batch_pipeline<uint32_t>(b, pts, link_leaves, s)
+                              : batch_pipeline<uint64_t>(b, pts, link_leaves, s);
+}
+
+void free_batch(tc_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->plan_copied) (void)hipEventDestroy(b->plan_copied);
+    if (b->h_plan) (void)hipHostFree(b->h_plan);
+    void* ptrs[] = {b->plan, b->seg_hist, b->lb, b->ctrs, b->ubase, b->node_base, b->totals, b->codes, b->alt,
+                    b->uniq, b->prefix_n, b->hll, b->hlr, b->left, b->parent, b->ec, b->offsets, b->nodes};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" int tc_batch_create(tc_ctx* ctx, uint32_t cap_clouds, uint32_t cap_points, int code_bits,
+                               tc_batch** out) {
+    if (!ctx || !out || cap_clouds == 0 || (code_bits != 30 && code_bits != 63)) return TC_ERR_INVALID;
+    if (cap_points > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    *out = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return TC_ERR_HIP;
+    tc_batch* b = new (std::nothrow) tc_batch();
+    if (!b) return TC_ERR_NOMEM;
+    b->ctx = ctx;
+    b->cap_clouds = cap_clouds;
+    b->cap_points = cap_points;
+    b->code_bits = code_bits;
+    b->block = ctx->sort_block;
+    b->kpt = ctx->sort_kpt;
+    b->tile = (uint32_t)(b->block * b->kpt);
+    b->cap_tiles = ceil_div_u32(cap_points, b->tile) + cap_clouds;
+    const size_t C = cap_clouds, N = cap_points ? cap_points : 1;
+    const size_t ks = code_bits == 30 ? 4 : 8;
+    const int dig = code_bits == 30 ? 4 : 8;
+    b->plan_words = 8 + 2 * (C + 1) + 2 * C;
+    // Node bound: per cloud it is tc_octree_node_bound(n_c); the sum over clouds
+    // is at most the bound for the total plus one root per cloud.
+    b->node_cap = tc_octree_node_bound(N, code_bits) + C;
+    bool ok = hipMalloc((void**)&b->plan, b->plan_words * 4) == hipSuccess &&
+              hipHostMalloc((void**)&b->h_plan, b->plan_words * 4) == hipSuccess &&
+              hipEventCreateWithFlags(&b->plan_copied, hipEventDisableTiming) == hipSuccess &&
+              hipMalloc((void**)&b->seg_hist, C * dig * 256 * 4) == hipSuccess &&
+              hipMalloc((void**)&b->lb, 2 * (size_t)b->cap_tiles * 256 * 4) == hipSuccess &&
+              hipMalloc((void**)&b->ctrs, 64) == hipSuccess &&
+              hipMalloc((void**)&b->ubase, (C + 1) * 4) == hipSuccess &&
+              hipMalloc((void**)&b->node_base, (C + 1) * 4) == hipSuccess &&
+              hipMalloc((void**)&b->totals, 16) == hipSuccess && hipMalloc(&b->codes, N * ks) == hipSuccess &&
+              hipMalloc(&b->alt, N * ks) == hipSuccess && hipMalloc(&b->uniq, N * ks) == hipSuccess &&
+              hipMalloc((void**)&b->prefix_n, N) == hipSuccess && hipMalloc((void**)&b->hll, N) == hipSuccess &&
+              hipMalloc((void**)&b->hlr, N) == hipSuccess && hipMalloc((void**)&b->left, N * 4) == hipSuccess &&
+              hipMalloc((void**)&b->parent, N * 4) == hipSuccess && hipMalloc((void**)&b->ec, N * 4) == hipSuccess &&
+              hipMalloc((void**)&b->offsets, (N + 1) * 4) == hipSuccess &&
+              hipMalloc((void**)&b->nodes, b->node_cap * sizeof(tc_oct_node)) == hipSuccess;
+    if (!ok) {
+        free_batch(b);
+        return TC_ERR_NOMEM;
+    }
+    std::memset(b->h_plan, 0, b->plan_words * 4);
+    if (hipMemset(b->plan, 0, b->plan_words * 4) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        free_batch(b);
+        return TC_ERR_HIP;
+    }
+    *out = b;
+    return TC_OK;
+}
+
+extern "C" int tc_batch_destroy(tc_batch* b) {
+    if (!b) return TC_ERR_INVALID;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    free_batch(b);
+    return TC_OK;
+}
+
+extern "C" int tc_batch_set(tc_batch* b, uint32_t n_clouds, const uint32_t* counts, const float* mins,
+                            const float* ranges, tc_stream stream) {
+    if (!b || (n_clouds && !counts) || n_clouds > b->cap_clouds) return TC_ERR_INVALID;
+    const uint32_t C = b->cap_clouds;
+    // The previous upload must have left the staging buffer.
+    TC_HIP(hipEventSynchronize(b->plan_copied));
+    uint32_t* h = b->h_plan;
+    uint32_t* pt_off = h + 8;
+    uint32_t* first = pt_off + (C + 1);
+    float* mn = reinterpret_cast<float*>(first + (C + 1));
+    float* rg = mn + C;
+    uint64_t pts = 0, tiles = 0;
+    for (uint32_t c = 0; c < n_clouds; ++c) {
+        pt_off[c] = (uint32_t)pts;
+        first[c] = (uint32_t)tiles;
+        pts += counts[c];
+        tiles += ceil_div_u32(counts[c], b->tile);
+        mn[c] = mins ? mins[c] : 0.0f;
+        rg[c] = ranges ? ranges[c] : 1024.0f;
+        if (!(rg[c] != 0.0f)) return TC_ERR_INVALID;
+    }
+    if (pts > b->cap_points) return TC_ERR_CAPACITY;
+    if (tiles > b->cap_tiles) return TC_ERR_CAPACITY;
+    pt_off[n_clouds] = (uint32_t)pts;
+    first[n_clouds] = (uint32_t)tiles;
+    h[0] = n_clouds;
+    h[1] = (uint32_t)tiles;
+    h[2] = (uint32_t)pts;
+    b->n_clouds = n_clouds;
+    b->n_points = (uint32_t)pts;
+    b->n_tiles = (uint32_t)tiles;
+    hipStream_t s = as_stream(stream);
+    TC_HIP(hipMemcpyAsync(b->plan, h, b->plan_words * 4, hipMemcpyHostToDevice, s));
+    TC_HIP(hipEventRecord(b->plan_copied, s));
+    return TC_OK;
+}
+
+extern "C" int tc_batch_build(tc_batch* b, const float* pts, int link_leaves, tc_stream stream) {
+    if (!b || (b->n_points && !pts)) return TC_ERR_INVALID;
+    if (reinterpret_cast<uintptr_t>(pts) & 15u) return TC_ERR_INVALID;
+    return run_pipeline(b, pts, link_leaves, as_stream(stream));
+}
+
+extern "C" int tc_batch_capture(tc_batch* b, const float* pts, int link_leaves, tc_stream stream) {
+    if (!b || !pts || (reinterpret_cast<uintptr_t>(pts) & 15u)) return TC_ERR_INVALID;
+    hipStream_t s = as_stream(stream);
+    if (!s) return TC_ERR_INVALID;  // the legacy stream cannot be captured
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    const bool prof = b->ctx->prof_on;
+    b->ctx->prof_on = false;  // no event records inside the graph
+    TC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int st = run_pipeline(b, pts, link_leaves, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    b->ctx->prof_on = prof;
+    if (st != TC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess || !g) return TC_ERR_HIP;
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) return TC_ERR_HIP;
+    b->graph_pts = pts;
+    b->graph_link = link_leaves;
+    return TC_OK;
+}
+
+extern "C" int tc_batch_replay(tc_batch* b, tc_stream stream) {
+    if (!b || !b->exec) return TC_ERR_INVALID;
+    ProfScope prof(b->ctx, as_stream(stream), K_SEG);
+    TC_HIP(hipGraphLaunch(b->exec, as_stream(stream)));
+    return TC_OK;
+}
+
+extern "C" int tc_batch_read_info(tc_batch* b, tc_cloud_info* info, tc_stream stream) {
+    if (!b || (b->n_clouds && !info)) return TC_ERR_INVALID;
+    TC_HIP(hipStreamSynchronize(as_stream(stream)));
+    const uint32_t C = b->n_clouds;
+    std::vector<uint32_t> ub(C + 1), nbase(C + 1);
+    TC_HIP(hipMemcpy(ub.data(), b->ubase, (C + 1) * 4, hipMemcpyDeviceToHost));
+    TC_HIP(hipMemcpy(nbase.data(), b->node_base, (C + 1) * 4, hipMemcpyDeviceToHost));
+    const uint32_t* pt_off = b->h_plan + 8;
+    for (uint32_t c = 0; c < C; ++c) {
+        info[c].point_offset = pt_off[c];
+        info[c].n_points = pt_off[c + 1] - pt_off[c];
+        info[c].unique_offset = ub[c];
+        info[c].n_unique = ub[c + 1] - ub[c];
+        info[c].node_offset = nbase[c];
+        info[c].n_nodes = nbase[c + 1] - nbase[c];
+    }
+    if (nbase[C] > b->node_cap) return TC_ERR_CAPACITY;
+    return TC_OK;
+}
+
+extern "C" int tc_batch_arrays(tc_batch* b, tc_batch_view* v) {
+    if (!b || !v) return TC_ERR_INVALID;
+    v->code_bits = b->code_bits;
+    v->n_clouds = b->n_clouds;
+    v->n_points = b->n_points;
+    v->sorted = b->codes;
+    v->unique = b->uniq;
+    v->prefix_n = b->prefix_n;
+    v->has_leaf_left = b->hll;
+    v->has_leaf_right = b->hlr;
+    v->left_child = b->left;
+    v->parent = b->parent;
+    v->edge_count = b->ec;
+    v->offsets = b->offsets;
+    v->nodes = b->nodes;
+    v->node_capacity = b->node_cap;
+    return TC_OK;
+}

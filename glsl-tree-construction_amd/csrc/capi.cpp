@@ -101,7 +101,7 @@ hipEvent_t tc_prof_event(tc_ctx* ctx) {
 
 extern "C" const char* tc_kernel_name(int id) {
     static const char* names[K_COUNT] = {"morton", "histogram", "binning", "unique", "scan", "radix_tree",
-                                         "edge_count", "edge_count_scan", "octree_emit", "octree_link",
+                                         "edge_count", "edge_count_scan", "octree", "octree_link",
                                          "fill", "decode", "segmented"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
@@ -319,36 +319,35 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     TC_TRY(tc_fill_u32(ctx, t->d_counts, 0u, 4, s));
     if (n == 0) return TC_OK;
     const uint32_t node_cap = (uint32_t)std::min<uint64_t>(t->node_capacity, 0xFFFFFFFFull);
+    // morton -> sort -> unique (count stays on the device) -> radix tree with
+    // fused edge counts -> exclusive scan -> octree.  Stages after unique read
+    // the unique count from d_counts; the scan always writes offsets[n_brt].
     if (t->code_bits == 30) {
         uint32_t* sorted = static_cast<uint32_t*>(t->sorted);
         uint32_t* uniq = static_cast<uint32_t*>(t->unique);
         TC_TRY(tc_morton30(ctx, pts, sorted, n, min_coord, range, stream));
         TC_TRY(tc_sort_keys_u32(ctx, sorted, static_cast<uint32_t*>(t->alt), n, stream));
         TC_TRY(tc_unique_u32(ctx, sorted, uniq, n, t->d_counts, stream));
-        TC_TRY(tc_build_radix_tree_u32(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
-                                       t->has_leaf_right, t->left_child, t->parent, stream));
-        if (n >= 2) {
-            TC_TRY(tc_edge_count_scan(ctx, t->prefix_n, t->parent, t->edge_count, t->offsets, n - 1, t->d_counts,
-                                      stream));
+        TC_TRY(tc_build_radix_tree_ec_u32(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
+                                          t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
+        TC_TRY(tc_exclusive_scan_u32(ctx, t->edge_count, t->offsets, n - 1, t->d_counts, stream));
+        if (n >= 2)
             TC_TRY(tc_build_octree_u32(ctx, t->nodes, node_cap, t->offsets, t->edge_count, uniq, t->prefix_n,
                                        t->parent, t->left_child, t->has_leaf_left, t->has_leaf_right, n - 1,
                                        t->d_counts, min_coord, range, link_leaves, stream));
-        }
     } else {
         uint64_t* sorted = static_cast<uint64_t*>(t->sorted);
         uint64_t* uniq = static_cast<uint64_t*>(t->unique);
         TC_TRY(tc_morton63(ctx, pts, sorted, n, min_coord, range, stream));
         TC_TRY(tc_sort_keys_u64(ctx, sorted, static_cast<uint64_t*>(t->alt), n, stream));
         TC_TRY(tc_unique_u64(ctx, sorted, uniq, n, t->d_counts, stream));
-        TC_TRY(tc_build_radix_tree_u64(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
-                                       t->has_leaf_right, t->left_child, t->parent, stream));
-        if (n >= 2) {
-            TC_TRY(tc_edge_count_scan(ctx, t->prefix_n, t->parent, t->edge_count, t->offsets, n - 1, t->d_counts,
-                                      stream));
+        TC_TRY(tc_build_radix_tree_ec_u64(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
+                                          t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
+        TC_TRY(tc_exclusive_scan_u32(ctx, t->edge_count, t->offsets, n - 1, t->d_counts, stream));
+        if (n >= 2)
             TC_TRY(tc_build_octree_u64(ctx, t->nodes, node_cap, t->offsets, t->edge_count, uniq, t->prefix_n,
                                        t->parent, t->left_child, t->has_leaf_left, t->has_leaf_right, n - 1,
                                        t->d_counts, min_coord, range, link_leaves, stream));
-        }
     }
     return TC_OK;
 }

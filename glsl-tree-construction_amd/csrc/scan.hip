@@ -1,5 +1,6 @@
 // scan.hip -- single-pass tile scans with decoupled look-back: unique
-// compaction, exclusive scan, and the fused edge-count + exclusive scan.
+// compaction, exclusive scan, the fused edge-count + exclusive scan, and the
+// segmented (many clouds per launch) forms.
 //
 // Reference: k_CountUnique, shader/unique.comp:31-45 and k_PartialSum,
 // shader/partial_sum.comp:23-39 are single-thread loops ([1,1,1] dispatches,
@@ -8,13 +9,21 @@
 // "Single-pass Parallel Prefix Scan with Decoupled Look-back" (2016).
 //
 // gfx950 design: persistent workgroups claim 4096-element tiles from an atomic
-// counter (so every waited-on tile is owned by a running workgroup); each
-// thread owns 16 consecutive elements; a block scan gives tile-local offsets;
-// wave 0 publishes the tile aggregate and looks back over 64 predecessors at a
-// time.  Tile status words are u64 [value:32 | flag:2 | epoch:30]; the epoch is
-// a device-side launch sequence number, so status words never need clearing
+// counter (so every waited-on tile is owned by a running workgroup).  A tile is
+// loaded with 16 B vector loads (each wave instruction reads 1 KiB contiguous)
+// into LDS and read back so each thread owns 16 consecutive elements; a block
+// scan gives tile-local offsets; wave 0 publishes the tile aggregate and looks
+// back over 64 predecessors at a time.  Results go back through LDS so the
+// stores are contiguous too (unique compacts the tile's keys in LDS first).
+// Tile status words are u64 [value:32 | flag:2 | epoch:30]; the epoch is a
+// device-side launch sequence number, so status words never need clearing
 // (also under graph replay).  The last workgroup to exit resets the tile
 // counter and advances the sequence number.
+//
+// Segmented modes run one scan over many concatenated clouds: an element that
+// starts a cloud always counts as a new unique key, and the exclusive prefix at
+// each cloud's first element is written to seg_out[cloud] (its base in the
+// compacted output).
 #include "tc_internal.hpp"
 
 namespace {
@@ -23,14 +32,38 @@ constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_IPT = 16;
 constexpr uint32_t SCAN_TILE = SCAN_BLOCK * SCAN_IPT;
 
-enum ScanMode { MODE_SCAN = 0, MODE_UNIQUE32 = 1, MODE_UNIQUE64 = 2, MODE_ECSCAN = 3 };
-
-// Scan control words (in ctx->small): [0] tile counter, [1] exit counter, [2] launch sequence.
+// Scan control words (in ctx->small): tile counter, exit counter, launch sequence.
 struct ScanCtl {
     uint32_t tile_ctr;
     uint32_t exit_ctr;
     uint32_t seq;
 };
+
+struct ScanArgs {
+    const void* in0;
+    const void* in1;
+    void* out0;
+    void* out1;
+    uint32_t* count_out;
+    uint32_t n_cap;
+    const uint32_t* d_n_keys;
+    const uint32_t* seg_off;  // segmented: [n_seg + 1] starts; total = seg_off[n_seg]
+    uint32_t n_seg_cap;
+    const uint32_t* d_n_seg;  // segmented: device segment count (NULL: n_seg_cap)
+    uint32_t* seg_out;        // segmented: [n_seg] exclusive prefix at each segment start
+    uint64_t* status;
+    ScanCtl* ctl;
+};
+
+constexpr bool is_seg(int m) { return m == MODE_SEG_UNIQUE32 || m == MODE_SEG_UNIQUE64 || m == MODE_SEG_SCAN; }
+constexpr bool is_unique(int m) {
+    return m == MODE_UNIQUE32 || m == MODE_UNIQUE64 || m == MODE_SEG_UNIQUE32 || m == MODE_SEG_UNIQUE64;
+}
+constexpr bool is_u64(int m) { return m == MODE_UNIQUE64 || m == MODE_SEG_UNIQUE64; }
+constexpr bool is_plain_values(int m) { return m == MODE_SCAN || m == MODE_SEG_SCAN; }
+
+template <int MODE>
+using ItemT = typename std::conditional<is_u64(MODE), uint64_t, uint32_t>::type;
 
 TC_DEV uint64_t st_word(uint32_t flag, uint32_t epoch, uint32_t value) {
     return ((uint64_t)((epoch << 2) | flag) << 32) | value;
@@ -63,8 +96,8 @@ TC_DEV uint32_t wave_lookback(const uint64_t* status, uint32_t tile, uint32_t ep
             }
             incl_mask = __ballot(f == 2u);
             const uint64_t nr = __ballot(f == 0u);
+            // Lanes up to and including the nearest inclusive predecessor.
             upto = incl_mask ? ((incl_mask & (~incl_mask + 1)) << 1) - 1 : ~0ull;
-            if (incl_mask == (1ull << 63)) upto = ~0ull;
             if ((nr & upto) == 0) break;
             __builtin_amdgcn_s_sleep(1);
         }
@@ -76,80 +109,125 @@ TC_DEV uint32_t wave_lookback(const uint64_t* status, uint32_t tile, uint32_t ep
     return excl;
 }
 
+// Largest c in [0, nseg-1] with off[c] <= i (skips empty segments).
+TC_DEV uint32_t find_seg(const uint32_t* off, uint32_t nseg, uint32_t i) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Coalesced tile load into LDS (tile-relative positions), then per-thread
+// blocked read-back is done by the caller.
+template <typename T>
+TC_DEV void load_tile(const T* __restrict__ in, uint32_t base, uint32_t cnt, T* s) {
+    constexpr int VEC = 16 / sizeof(T);
+    if (cnt == SCAN_TILE) {
+        using V = typename std::conditional<sizeof(T) == 4, uint4, ulonglong2>::type;
+        const V* vin = reinterpret_cast<const V*>(in + base);
+        V* vs = reinterpret_cast<V*>(s);
+#pragma unroll
+        for (int r = 0; r < (int)(SCAN_TILE / VEC / SCAN_BLOCK); ++r) vs[r * SCAN_BLOCK + threadIdx.x] = vin[r * SCAN_BLOCK + threadIdx.x];
+    } else {
+        for (uint32_t k = threadIdx.x; k < cnt; k += SCAN_BLOCK) s[k] = in[base + k];
+    }
+}
+
 template <int MODE>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const void* __restrict__ in0,
-                                                           const void* __restrict__ in1,
-                                                           void* __restrict__ out0, void* __restrict__ out1,
-                                                           uint32_t* __restrict__ count_out, uint32_t n_cap,
-                                                           const uint32_t* __restrict__ d_n_keys,
-                                                           uint64_t* __restrict__ status,
-                                                           ScanCtl* __restrict__ ctl) {
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
+    using T = ItemT<MODE>;
     __shared__ uint32_t s_tmp[SCAN_BLOCK / 64 + 2];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_excl;
+    __shared__ __attribute__((aligned(16))) T s_items[SCAN_TILE];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6;
-    const uint32_t epoch = ld_acquire_agent(&ctl->seq) & 0x3FFFFFFFu;
+    const uint32_t epoch = ld_acquire_agent(&a.ctl->seq) & 0x3FFFFFFFu;
 
-    uint32_t n = n_cap;
-    if (d_n_keys) {
-        const uint32_t u = *d_n_keys;
-        uint32_t m = (MODE == MODE_SCAN || MODE == MODE_ECSCAN) ? (u > 0 ? u - 1 : 0) : u;
-        n = m < n_cap ? m : n_cap;
+    uint32_t n = a.n_cap;
+    uint32_t nseg = 0;
+    if (is_seg(MODE)) {
+        nseg = a.d_n_seg ? *a.d_n_seg : a.n_seg_cap;
+        if (nseg > a.n_seg_cap) nseg = a.n_seg_cap;
+        n = nseg ? a.seg_off[nseg] : 0u;
+        if (n > a.n_cap) n = a.n_cap;
+    } else if (a.d_n_keys) {
+        const uint32_t u = *a.d_n_keys;
+        const uint32_t m = (MODE == MODE_SCAN || MODE == MODE_ECSCAN) ? (u > 0 ? u - 1 : 0) : u;
+        n = m < a.n_cap ? m : a.n_cap;
     }
     const uint32_t num_tiles = n ? ceil_div_u32(n, SCAN_TILE) : 1u;
+    // Unaligned inputs (tile starts not 16 B aligned) take the element path.
+    const bool aligned = (reinterpret_cast<uintptr_t>(a.in0) & 15u) == 0;
 
     while (true) {
-        if (tid == 0) s_tile = atomicAdd(&ctl->tile_ctr, 1u);
+        if (tid == 0) s_tile = atomicAdd(&a.ctl->tile_ctr, 1u);
         __syncthreads();
         const uint32_t tile = s_tile;
-        __syncthreads();
         if (tile >= num_tiles) break;
 
         const uint32_t base = tile * SCAN_TILE;
-        const uint32_t i0 = base + tid * SCAN_IPT;
+        const uint32_t cnt = n > base ? (n - base < SCAN_TILE ? n - base : SCAN_TILE) : 0u;
+        const uint32_t r0 = tid * SCAN_IPT;  // tile-relative index of this thread's first element
+        const uint32_t i0 = base + r0;
         uint32_t val[SCAN_IPT];
-        uint64_t key64[MODE == MODE_UNIQUE64 ? SCAN_IPT : 1];
-        uint32_t key32[MODE == MODE_UNIQUE32 ? SCAN_IPT : 1];
+        T item[is_unique(MODE) ? SCAN_IPT : 1];
+        uint32_t start_mask = 0;  // bit j: element i0 + j starts a segment
+        uint32_t seg_first = 0;
 
-        if (MODE == MODE_SCAN) {
-            const uint32_t* in = static_cast<const uint32_t*>(in0);
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) val[j] = (i0 + j < n) ? in[i0 + j] : 0u;
-        } else if (MODE == MODE_UNIQUE32) {
-            const uint32_t* in = static_cast<const uint32_t*>(in0);
-            uint32_t prev = (i0 > 0 && i0 - 1 < n) ? in[i0 - 1] : 0u;
+        if (MODE != MODE_ECSCAN) {
+            if (aligned) load_tile<T>(static_cast<const T*>(a.in0), base, cnt, s_items);
+            else
+                for (uint32_t k = tid; k < cnt; k += SCAN_BLOCK) s_items[k] = static_cast<const T*>(a.in0)[base + k];
+            __syncthreads();
+        }
+        if (is_seg(MODE) && nseg > 0 && i0 < n) {
+            uint32_t seg = find_seg(a.seg_off, nseg, i0);
+            seg_first = seg;
+            uint32_t seg_start = a.seg_off[seg], seg_next = a.seg_off[seg + 1];
 #pragma unroll
             for (int j = 0; j < SCAN_IPT; ++j) {
                 const uint32_t i = i0 + j;
-                const uint32_t k = i < n ? in[i] : 0u;
-                key32[j] = k;
-                val[j] = (i < n && (i == 0 || k != prev)) ? 1u : 0u;
-                prev = k;
+                if (i >= n) break;
+                while (i >= seg_next && seg + 1 < nseg) {
+                    ++seg;
+                    seg_start = a.seg_off[seg];
+                    seg_next = a.seg_off[seg + 1];
+                }
+                if (i == seg_start) start_mask |= 1u << j;
             }
-        } else if (MODE == MODE_UNIQUE64) {
-            const uint64_t* in = static_cast<const uint64_t*>(in0);
-            uint64_t prev = (i0 > 0 && i0 - 1 < n) ? in[i0 - 1] : 0ull;
+        }
+
+        if (is_plain_values(MODE)) {
+#pragma unroll
+            for (int j = 0; j < SCAN_IPT; ++j) val[j] = (r0 + j < cnt) ? (uint32_t)s_items[r0 + j] : 0u;
+        } else if (is_unique(MODE)) {
+            T prev = (T)0;
+            if (r0 < cnt) prev = r0 > 0 ? s_items[r0 - 1] : (base > 0 ? static_cast<const T*>(a.in0)[base - 1] : (T)0);
 #pragma unroll
             for (int j = 0; j < SCAN_IPT; ++j) {
-                const uint32_t i = i0 + j;
-                const uint64_t k = i < n ? in[i] : 0ull;
-                key64[j] = k;
-                val[j] = (i < n && (i == 0 || k != prev)) ? 1u : 0u;
+                const uint32_t r = r0 + j;
+                const T k = r < cnt ? s_items[r] : (T)0;
+                const bool starts = (base + r == 0) || ((start_mask >> j) & 1u);
+                item[j] = k;
+                val[j] = (r < cnt && (starts || k != prev)) ? 1u : 0u;
                 prev = k;
             }
         } else {  // MODE_ECSCAN: edge count of radix node i, root entry 1
-            const uint8_t* pn = static_cast<const uint8_t*>(in0);
-            const int32_t* par = static_cast<const int32_t*>(in1);
+            const uint8_t* pn = static_cast<const uint8_t*>(a.in0);
+            const int32_t* par = static_cast<const int32_t*>(a.in1);
 #pragma unroll
             for (int j = 0; j < SCAN_IPT; ++j) {
-                const uint32_t i = i0 + j;
+                const uint32_t g = i0 + j;
                 uint32_t e = 0;
-                if (i < n) {
-                    uint32_t p = (uint32_t)par[i];
+                if (g < n) {
+                    uint32_t p = (uint32_t)par[g];
                     p = p < n ? p : 0u;  // parent indices of a valid tree are < n
-                    e = (i == 0) ? 1u : (uint32_t)(pn[i] / 3 - pn[p] / 3);
+                    e = (g == 0) ? 1u : (uint32_t)(pn[g] / 3 - pn[p] / 3);
                 }
                 val[j] = e;
             }
@@ -165,46 +243,63 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const void* __restric
             const uint32_t lane = lane_id();
             if (tile == 0) {
                 if (lane == 0) {
-                    st_agent64(&status[0], st_word(2u, epoch, block_total));
+                    st_agent64(&a.status[0], st_word(2u, epoch, block_total));
                     s_excl = 0;
                 }
             } else {
-                if (lane == 0) st_agent64(&status[tile], st_word(1u, epoch, block_total));
-                const uint32_t ex = wave_lookback(status, tile, epoch);
+                if (lane == 0) st_agent64(&a.status[tile], st_word(1u, epoch, block_total));
+                const uint32_t ex = wave_lookback(a.status, tile, epoch);
                 if (lane == 0) {
-                    st_agent64(&status[tile], st_word(2u, epoch, ex + block_total));
+                    st_agent64(&a.status[tile], st_word(2u, epoch, ex + block_total));
                     s_excl = ex;
                 }
             }
         }
         __syncthreads();
         const uint32_t tile_excl = s_excl;
-        uint32_t run = tile_excl + texcl;
 
-        if (MODE == MODE_SCAN) {
-            uint32_t* out = static_cast<uint32_t*>(out0);
+        // Segment bases: the exclusive prefix at each segment's first element.
+        if (is_seg(MODE) && start_mask) {
+            uint32_t seg = seg_first;
+            uint32_t run = tile_excl + texcl;
 #pragma unroll
             for (int j = 0; j < SCAN_IPT; ++j) {
-                if (i0 + j < n) out[i0 + j] = run;
+                if ((start_mask >> j) & 1u) {
+                    // the last segment starting here (earlier equal offsets are empty clouds)
+                    while (seg + 1 < nseg && a.seg_off[seg + 1] <= i0 + j) ++seg;
+                    a.seg_out[seg] = run;
+                }
                 run += val[j];
             }
-        } else if (MODE == MODE_UNIQUE32) {
-            uint32_t* out = static_cast<uint32_t*>(out0);
+        }
+
+        if (is_plain_values(MODE)) {
+            // Exclusive values through LDS, then contiguous stores.
+            uint32_t run = tile_excl + texcl;
+            uint32_t* s_out = reinterpret_cast<uint32_t*>(s_items);
 #pragma unroll
             for (int j = 0; j < SCAN_IPT; ++j) {
-                if (val[j]) out[run] = key32[j];
+                s_out[r0 + j] = run;
                 run += val[j];
             }
-        } else if (MODE == MODE_UNIQUE64) {
-            uint64_t* out = static_cast<uint64_t*>(out0);
+            __syncthreads();
+            uint32_t* out = static_cast<uint32_t*>(a.out0);
+            for (uint32_t k = tid; k < cnt; k += SCAN_BLOCK) out[base + k] = s_out[k];
+        } else if (is_unique(MODE)) {
+            // Compact the tile's kept keys in LDS, then store them contiguously.
+            uint32_t run = texcl;
 #pragma unroll
             for (int j = 0; j < SCAN_IPT; ++j) {
-                if (val[j]) out[run] = key64[j];
+                if (val[j]) s_items[run] = item[j];
                 run += val[j];
             }
+            __syncthreads();
+            T* out = static_cast<T*>(a.out0);
+            for (uint32_t k = tid; k < block_total; k += SCAN_BLOCK) out[tile_excl + k] = s_items[k];
         } else {
-            uint32_t* ec = static_cast<uint32_t*>(out0);
-            uint32_t* off = static_cast<uint32_t*>(out1);
+            uint32_t run = tile_excl + texcl;
+            uint32_t* ec = static_cast<uint32_t*>(a.out0);
+            uint32_t* off = static_cast<uint32_t*>(a.out1);
 #pragma unroll
             for (int j = 0; j < SCAN_IPT; ++j) {
                 if (i0 + j < n) {
@@ -216,8 +311,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const void* __restric
         }
         if (tile == num_tiles - 1 && tid == SCAN_BLOCK - 1) {
             const uint32_t total = tile_excl + block_total;
-            if (MODE == MODE_SCAN || MODE == MODE_ECSCAN) static_cast<uint32_t*>(MODE == MODE_SCAN ? out0 : out1)[n] = total;
-            if (count_out) *count_out = total;
+            if (is_plain_values(MODE)) static_cast<uint32_t*>(a.out0)[n] = total;
+            if (MODE == MODE_ECSCAN) static_cast<uint32_t*>(a.out1)[n] = total;
+            if (a.count_out) *a.count_out = total;
         }
         __syncthreads();
     }
@@ -225,13 +321,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const void* __restric
     // Exit protocol: the last workgroup out resets the counter for the next launch.
     if (tid == 0) {
         __threadfence();
-        const uint32_t t = atomicAdd(&ctl->exit_ctr, 1u);
+        const uint32_t t = atomicAdd(&a.ctl->exit_ctr, 1u);
         if (t == gridDim.x - 1) {
-            st_agent(&ctl->tile_ctr, 0u);
-            st_agent(&ctl->exit_ctr, 0u);
-            st_agent(&ctl->seq, (epoch + 1u) & 0x3FFFFFFFu);
+            st_agent(&a.ctl->tile_ctr, 0u);
+            st_agent(&a.ctl->exit_ctr, 0u);
+            st_agent(&a.ctl->seq, (epoch + 1u) & 0x3FFFFFFFu);
         }
     }
+}
+
+template <int MODE>
+void launch(uint32_t grid, hipStream_t s, const ScanArgs& a) {
+    hipLaunchKernelGGL(k_scan_tiles<MODE>, dim3(grid), dim3(SCAN_BLOCK), 0, s, a);
 }
 
 }  // namespace
@@ -244,36 +345,27 @@ int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block) {
     return (int)(blocks < cap ? blocks : cap);
 }
 
-int tc_launch_scan_tiles(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0, void* out1,
-                         uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, hipStream_t s) {
+int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0, void* out1,
+                   uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, const uint32_t* seg_off,
+                   uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s) {
     const uint32_t tiles = ceil_div_u32(n ? n : 1, SCAN_TILE);
-    uint64_t* status = ctx->scan_status;
-    if (!status) return TC_ERR_NOMEM;
+    if (!ctx->scan_status) return TC_ERR_NOMEM;
     // Status words for up to 2^30 / 4096 tiles are preallocated with the context.
     if (tiles > (1u << 18)) return TC_ERR_TOO_LARGE;
-    ScanCtl* ctl = reinterpret_cast<ScanCtl*>(ctx->small + SMALL_SCAN_CTR);
-    const uint32_t cap = (uint32_t)ctx->num_cus * 8;
+    ScanArgs a{in0, in1, out0, out1, count_out, n, d_n_keys, seg_off, n_seg_cap, d_n_seg, seg_out,
+               ctx->scan_status, reinterpret_cast<ScanCtl*>(ctx->small + SMALL_SCAN_CTR)};
+    const uint32_t cap = (uint32_t)ctx->num_cus * 4;
     const uint32_t grid = tiles < cap ? tiles : cap;
-    ProfScope prof(ctx, s, mode == MODE_SCAN ? K_SCAN : mode == MODE_ECSCAN ? K_ECSCAN : K_UNIQUE);
+    ProfScope prof(ctx, s, is_seg(mode) ? K_SEG : mode == MODE_SCAN ? K_SCAN : mode == MODE_ECSCAN ? K_ECSCAN : K_UNIQUE);
     switch (mode) {
-        case MODE_SCAN:
-            hipLaunchKernelGGL(k_scan_tiles<MODE_SCAN>, dim3(grid), dim3(SCAN_BLOCK), 0, s, in0, in1, out0, out1,
-                               count_out, n, d_n_keys, status, ctl);
-            break;
-        case MODE_UNIQUE32:
-            hipLaunchKernelGGL(k_scan_tiles<MODE_UNIQUE32>, dim3(grid), dim3(SCAN_BLOCK), 0, s, in0, in1, out0,
-                               out1, count_out, n, d_n_keys, status, ctl);
-            break;
-        case MODE_UNIQUE64:
-            hipLaunchKernelGGL(k_scan_tiles<MODE_UNIQUE64>, dim3(grid), dim3(SCAN_BLOCK), 0, s, in0, in1, out0,
-                               out1, count_out, n, d_n_keys, status, ctl);
-            break;
-        case MODE_ECSCAN:
-            hipLaunchKernelGGL(k_scan_tiles<MODE_ECSCAN>, dim3(grid), dim3(SCAN_BLOCK), 0, s, in0, in1, out0,
-                               out1, count_out, n, d_n_keys, status, ctl);
-            break;
-        default:
-            return TC_ERR_INVALID;
+        case MODE_SCAN: launch<MODE_SCAN>(grid, s, a); break;
+        case MODE_UNIQUE32: launch<MODE_UNIQUE32>(grid, s, a); break;
+        case MODE_UNIQUE64: launch<MODE_UNIQUE64>(grid, s, a); break;
+        case MODE_ECSCAN: launch<MODE_ECSCAN>(grid, s, a); break;
+        case MODE_SEG_UNIQUE32: launch<MODE_SEG_UNIQUE32>(grid, s, a); break;
+        case MODE_SEG_UNIQUE64: launch<MODE_SEG_UNIQUE64>(grid, s, a); break;
+        case MODE_SEG_SCAN: launch<MODE_SEG_SCAN>(grid, s, a); break;
+        default: return TC_ERR_INVALID;
     }
     TC_LAUNCH_CHECK();
     return TC_OK;
@@ -283,24 +375,24 @@ extern "C" int tc_unique_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out,
                              uint32_t* d_n_unique, tc_stream stream) {
     if (!ctx || !d_n_unique || (n && (!sorted || !out))) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
-    return tc_launch_scan_tiles(ctx, MODE_UNIQUE32, sorted, nullptr, out, nullptr, d_n_unique, n, nullptr,
-                                as_stream(stream));
+    return tc_launch_scan(ctx, MODE_UNIQUE32, sorted, nullptr, out, nullptr, d_n_unique, n, nullptr, nullptr, 0,
+                          nullptr, nullptr, as_stream(stream));
 }
 
 extern "C" int tc_unique_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out, uint32_t n,
                              uint32_t* d_n_unique, tc_stream stream) {
     if (!ctx || !d_n_unique || (n && (!sorted || !out))) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
-    return tc_launch_scan_tiles(ctx, MODE_UNIQUE64, sorted, nullptr, out, nullptr, d_n_unique, n, nullptr,
-                                as_stream(stream));
+    return tc_launch_scan(ctx, MODE_UNIQUE64, sorted, nullptr, out, nullptr, d_n_unique, n, nullptr, nullptr, 0,
+                          nullptr, nullptr, as_stream(stream));
 }
 
 extern "C" int tc_exclusive_scan_u32(tc_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                                      const uint32_t* d_n_keys, tc_stream stream) {
     if (!ctx || !out || (n && !in)) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
-    return tc_launch_scan_tiles(ctx, MODE_SCAN, in, nullptr, out, nullptr, nullptr, n, d_n_keys,
-                                as_stream(stream));
+    return tc_launch_scan(ctx, MODE_SCAN, in, nullptr, out, nullptr, nullptr, n, d_n_keys, nullptr, 0, nullptr,
+                          nullptr, as_stream(stream));
 }
 
 extern "C" int tc_edge_count_scan(tc_ctx* ctx, const uint8_t* prefix_n, const int32_t* parent,
@@ -308,6 +400,6 @@ extern "C" int tc_edge_count_scan(tc_ctx* ctx, const uint8_t* prefix_n, const in
                                   const uint32_t* d_n_keys, tc_stream stream) {
     if (!ctx || !offsets || (n_brt && (!prefix_n || !parent || !edge_count))) return TC_ERR_INVALID;
     if (n_brt > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
-    return tc_launch_scan_tiles(ctx, MODE_ECSCAN, prefix_n, parent, edge_count, offsets, nullptr, n_brt,
-                                d_n_keys, as_stream(stream));
+    return tc_launch_scan(ctx, MODE_ECSCAN, prefix_n, parent, edge_count, offsets, nullptr, n_brt, d_n_keys,
+                          nullptr, 0, nullptr, nullptr, as_stream(stream));
 }

@@ -22,7 +22,7 @@
 //    read with agent-scope atomics, since the 8 XCDs' L2 caches are not
 //    coherent with each other; one thread per digit walks back.  Each pass
 //    clears the other look-back region for the next pass.
-#include "tc_internal.hpp"
+#include "sort_core.hpp"
 
 namespace {
 
@@ -104,169 +104,26 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
     if (tid == 0) st_agent(done, 0u);
 }
 
-// Peer mask: lanes of this wave whose digit equals mine, among `live` lanes.
-TC_DEV uint64_t match_digit8(uint32_t d, uint64_t live) {
-    uint64_t peers = live;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
-    }
-    return peers;
-}
-
-template <typename K>
-TC_DEV uint32_t digit_of(K k, uint32_t shift) {
-    return (uint32_t)((k >> shift) & 255u);
-}
-
 template <typename K, bool PAIRS, int BLOCK, int KPT>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const K* __restrict__ sk, K* __restrict__ dk, const uint32_t* __restrict__ sv,
     uint32_t* __restrict__ dv, uint32_t n, uint32_t shift, const uint32_t* __restrict__ digit_prefix,
     uint32_t* __restrict__ lb_cur, uint32_t* __restrict__ lb_next, uint32_t* __restrict__ tile_ctr) {
-    constexpr int WAVES = BLOCK / 64;
-    constexpr uint32_t TILE = BLOCK * KPT;
-    constexpr uint32_t WAVE_TILE = 64 * KPT;
-    static_assert(BLOCK >= 256, "one thread per digit");
-    constexpr size_t STAGE_BYTES = TILE * sizeof(K);
-
-    __shared__ uint32_t s_whist[WAVES * 256];
-    __shared__ uint32_t s_start[256];
-    __shared__ uint32_t s_gbase[256];
-    __shared__ uint32_t s_tmp[WAVES + 2];
-    __shared__ uint32_t s_tile;
-    __shared__ __attribute__((aligned(16))) unsigned char s_stage[STAGE_BYTES];
-    K* s_keys = reinterpret_cast<K*>(s_stage);
-    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_stage);  // reused after the keys are out
-
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
-    const uint32_t wave = tid >> 6;
-
-    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_whist[i] = 0u;
-    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t base = tile * TILE;
-    const bool full = base + TILE <= n;
-    const uint32_t wbase = base + wave * WAVE_TILE + lane;
-
-    K keys[KPT];
-    uint32_t vals[KPT];
-    uint32_t pos[KPT];
-    if (full) {
-#pragma unroll
-        for (int i = 0; i < KPT; ++i) keys[i] = sk[wbase + i * 64];
-        if (PAIRS) {
-#pragma unroll
-            for (int i = 0; i < KPT; ++i) vals[i] = sv[wbase + i * 64];
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < KPT; ++i) {
-            const uint32_t idx = wbase + i * 64;
-            keys[i] = idx < n ? sk[idx] : (K)0;
-            if (PAIRS) vals[i] = idx < n ? sv[idx] : 0u;
-        }
-    }
-    if (lb_next) {
-        for (uint32_t i = tid; i < 256; i += BLOCK) lb_next[tile * 256 + i] = 0u;
-    }
-
-    // Stable wave-level ranking.  Lanes of one wave touch this wave's counters
-    // only; LDS operations of a wave complete in order, so each iteration sees
-    // the previous iteration's leader updates.
-    volatile uint32_t* wh = s_whist + wave * 256;
-#pragma unroll
-    for (int i = 0; i < KPT; ++i) {
-        const uint32_t d = digit_of<K>(keys[i], shift);
-        const bool valid = full || (wbase + i * 64 < n);
-        const uint64_t peers = match_digit8(d, __ballot(valid));
-        const uint32_t lt = mbcnt64(peers);
-        const uint32_t before = wh[d];
-        pos[i] = before + lt;
-        if (valid && lt == 0) wh[d] = before + (uint32_t)__popcll(peers);
-    }
-    __syncthreads();
-
-    // Per digit: exclusive prefix over waves, tile total, publish aggregate.
-    uint32_t total_d = 0;
-    if (tid < 256) {
-        uint32_t run = 0;
-#pragma unroll
-        for (int w = 0; w < WAVES; ++w) {
-            const uint32_t c = s_whist[w * 256 + tid];
-            s_whist[w * 256 + tid] = run;
-            run += c;
-        }
-        total_d = run;
-        st_agent(&lb_cur[tile * 256 + tid], (tile == 0 ? LB_INCLUSIVE : LB_AGGREGATE) | (total_d << 2));
-    }
-    uint32_t tile_total;
-    const uint32_t start = block_exclusive_scan<BLOCK>(total_d, s_tmp, &tile_total);
-    if (tid < 256) s_start[tid] = start;
-    __syncthreads();
-
-    // Stage keys in LDS in (digit, original order) order.
-#pragma unroll
-    for (int i = 0; i < KPT; ++i) {
-        const bool valid = full || (wbase + i * 64 < n);
-        if (valid) {
-            const uint32_t d = digit_of<K>(keys[i], shift);
-            const uint32_t lp = s_start[d] + s_whist[wave * 256 + d] + pos[i];
-            s_keys[lp] = keys[i];
-            pos[i] = lp;
-        }
-    }
-
-    // Decoupled look-back, one thread per digit.
-    if (tid < 256) {
-        uint32_t excl = 0;
-        if (tile > 0) {
-            int32_t k = (int32_t)tile - 1;
-            while (true) {
-                const uint32_t v = ld_acquire_agent(&lb_cur[(uint32_t)k * 256 + tid]);
-                const uint32_t f = v & LB_FLAG_MASK;
-                if (f == LB_NOT_READY) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += v >> 2;
-                if (f == LB_INCLUSIVE) break;
-                --k;
-            }
-            st_agent(&lb_cur[tile * 256 + tid], LB_INCLUSIVE | ((excl + total_d) << 2));
-        }
-        s_gbase[tid] = digit_prefix[tid] + excl - start;
-    }
-    __syncthreads();
-
-    const uint32_t cnt = full ? TILE : n - base;
-    uint32_t dst[KPT];
-#pragma unroll
-    for (int i = 0; i < KPT; ++i) {
-        const uint32_t j = tid + i * BLOCK;
-        if (j < cnt) {
-            const K key = s_keys[j];
-            dst[i] = s_gbase[digit_of<K>(key, shift)] + j;
-            dk[dst[i]] = key;
-        }
-    }
-    if (PAIRS) {
+    using Sh = OnesweepShared<K, BLOCK, KPT>;
+    __shared__ Sh sm;
+    const uint32_t num_tiles = ceil_div_u32(n, Sh::TILE);
+    // Tiles are claimed in order from a counter, so every tile a look-back
+    // waits on belongs to a workgroup that is already running.
+    while (true) {
+        if (threadIdx.x == 0) sm.tile = atomicAdd(tile_ctr, 1u);
         __syncthreads();
-#pragma unroll
-        for (int i = 0; i < KPT; ++i) {
-            const bool valid = full || (wbase + i * 64 < n);
-            if (valid) s_vals[pos[i]] = vals[i];
-        }
+        const uint32_t tile = sm.tile;
+        if (tile >= num_tiles) break;
+        const uint32_t base = tile * Sh::TILE;
+        const uint32_t cnt = n - base < Sh::TILE ? n - base : Sh::TILE;
+        onesweep_tile<K, PAIRS, BLOCK, KPT>(sm, sk, dk, sv, dv, base, cnt, shift, tile, 0u, digit_prefix, 0u,
+                                            lb_cur, lb_next);
         __syncthreads();
-#pragma unroll
-        for (int i = 0; i < KPT; ++i) {
-            const uint32_t j = tid + i * BLOCK;
-            if (j < cnt) dv[dst[i]] = s_vals[j];
-        }
     }
 }
 

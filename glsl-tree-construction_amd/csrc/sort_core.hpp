@@ -1,0 +1,189 @@
+// sort_core.hpp -- the OneSweep tile step shared by the plain sort (sort.hip)
+// and the segmented batch sort (batch.hip).
+//
+// One tile = BLOCK*KPT keys.  Wave w owns keys [w*64*KPT, (w+1)*64*KPT) of the
+// tile, lane-striped (key i*64 + lane), so a wave's processing order (i, lane)
+// is the keys' index order and the rank below is stable.
+#pragma once
+
+#include "tc_internal.hpp"
+
+// Peer mask: lanes of this wave whose 8-bit digit equals mine, among `live`
+// lanes (8 ballots; the reference's 32-lane version is
+// shader/new_radix_sort.comp:147-152).  Must be called with all lanes active.
+TC_DEV uint64_t match_digit8(uint32_t d, uint64_t live) {
+    uint64_t peers = live;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+template <typename K>
+TC_DEV uint32_t digit_of(K k, uint32_t shift) {
+    return (uint32_t)((k >> shift) & 255u);
+}
+
+template <typename K, int BLOCK, int KPT>
+struct OnesweepShared {
+    static constexpr int WAVES = BLOCK / 64;
+    static constexpr uint32_t TILE = BLOCK * KPT;
+    uint32_t whist[WAVES * 256];
+    uint32_t start[256];
+    uint32_t gbase[256];
+    uint32_t tmp[WAVES + 2];
+    uint32_t tile;
+    uint32_t aux;
+    alignas(16) unsigned char stage[TILE * sizeof(K)];
+};
+
+// One digit-binning tile step (shader/new_radix_sort.comp:116-314):
+//   keys sk[base, base+cnt) (cnt <= TILE) go to
+//   dk[prefix[d] + add + (#digit-d keys of earlier tiles of this segment) + rank].
+// `tile` indexes the look-back rows (256 u32 words per tile) of lb_cur; the
+// look-back walks down to `first`, the first tile of the segment, which
+// publishes its counts as inclusive.  lb_next (may be NULL) is the next pass's
+// look-back row of this tile, cleared here.
+template <typename K, bool PAIRS, int BLOCK, int KPT>
+TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict__ sk, K* __restrict__ dk,
+                          const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
+                          uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first,
+                          const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
+                          uint32_t* __restrict__ lb_next) {
+    constexpr int WAVES = BLOCK / 64;
+    constexpr uint32_t TILE = BLOCK * KPT;
+    constexpr uint32_t WAVE_TILE = 64 * KPT;
+    static_assert(BLOCK >= 256, "one thread per digit");
+    K* s_keys = reinterpret_cast<K*>(sm.stage);
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(sm.stage);  // reused once the keys are out
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    const bool full = cnt == TILE;
+    const uint32_t wofs = wave * WAVE_TILE + lane;  // tile-relative index of key 0 of this lane
+
+    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) sm.whist[i] = 0u;
+
+    K keys[KPT];
+    uint32_t vals[KPT];
+    uint32_t pos[KPT];
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) keys[i] = sk[base + wofs + i * 64];
+        if (PAIRS) {
+#pragma unroll
+            for (int i = 0; i < KPT; ++i) vals[i] = sv[base + wofs + i * 64];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) {
+            const uint32_t r = wofs + i * 64;
+            keys[i] = r < cnt ? sk[base + r] : (K)0;
+            if (PAIRS) vals[i] = r < cnt ? sv[base + r] : 0u;
+        }
+    }
+    if (lb_next) {
+        for (uint32_t i = tid; i < 256; i += BLOCK) lb_next[tile * 256 + i] = 0u;
+    }
+    __syncthreads();  // counters cleared
+
+    // Stable wave-level ranking.  Each wave touches only its own counters; LDS
+    // operations of one wave execute in program order, so iteration i+1
+    // observes the leader updates of iteration i.  The signal fence keeps the
+    // compiler from moving the LDS accesses across iterations (it emits no
+    // instruction; a volatile pointer here would turn them into flat accesses).
+    uint32_t* wh = sm.whist + wave * 256;
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const uint32_t d = digit_of<K>(keys[i], shift);
+        const bool valid = full || (wofs + i * 64 < cnt);
+        const uint64_t peers = match_digit8(d, __ballot(valid));
+        const uint32_t lt = mbcnt64(peers);
+        const uint32_t before = wh[d];
+        pos[i] = before + lt;
+        if (valid && lt == 0) wh[d] = before + (uint32_t)__popcll(peers);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+    __syncthreads();
+
+    // Per digit: exclusive prefix over waves, tile total, publish aggregate.
+    uint32_t total_d = 0;
+    if (tid < 256) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = sm.whist[w * 256 + tid];
+            sm.whist[w * 256 + tid] = run;
+            run += c;
+        }
+        total_d = run;
+        st_agent(&lb_cur[tile * 256 + tid], (tile == first ? LB_INCLUSIVE : LB_AGGREGATE) | (total_d << 2));
+    }
+    uint32_t tile_total;
+    const uint32_t start = block_exclusive_scan<BLOCK>(total_d, sm.tmp, &tile_total);
+    if (tid < 256) sm.start[tid] = start;
+    __syncthreads();
+
+    // Stage keys in LDS in (digit, original order) order.
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const bool valid = full || (wofs + i * 64 < cnt);
+        if (valid) {
+            const uint32_t d = digit_of<K>(keys[i], shift);
+            const uint32_t lp = sm.start[d] + sm.whist[wave * 256 + d] + pos[i];
+            s_keys[lp] = keys[i];
+            pos[i] = lp;
+        }
+    }
+
+    // Decoupled look-back, one thread per digit, down to the segment's first tile.
+    if (tid < 256) {
+        uint32_t excl = 0;
+        if (tile != first) {
+            uint32_t k = tile - 1;
+            while (true) {
+                const uint32_t v = ld_acquire_agent(&lb_cur[k * 256 + tid]);
+                const uint32_t f = v & LB_FLAG_MASK;
+                if (f == LB_NOT_READY) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v >> 2;
+                if (f == LB_INCLUSIVE) break;
+                --k;
+            }
+            st_agent(&lb_cur[tile * 256 + tid], LB_INCLUSIVE | ((excl + total_d) << 2));
+        }
+        sm.gbase[tid] = prefix[tid] + add + excl - start;
+    }
+    __syncthreads();
+
+    uint32_t dst[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const uint32_t j = tid + i * BLOCK;
+        if (j < cnt) {
+            const K key = s_keys[j];
+            dst[i] = sm.gbase[digit_of<K>(key, shift)] + j;
+            dk[dst[i]] = key;
+        }
+    }
+    if (PAIRS) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) {
+            const bool valid = full || (wofs + i * 64 < cnt);
+            if (valid) s_vals[pos[i]] = vals[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) {
+            const uint32_t j = tid + i * BLOCK;
+            if (j < cnt) dv[dst[i]] = s_vals[j];
+        }
+    }
+}

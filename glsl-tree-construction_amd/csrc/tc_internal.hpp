@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/tc.h"
@@ -126,11 +127,22 @@ __host__ __device__ static inline uint32_t ceil_div_u32(uint64_t a, uint64_t b) 
     return (uint32_t)((a + b - 1) / b);
 }
 
+// Modes of the single-pass tile scan (scan.hip).
+enum ScanMode {
+    MODE_SCAN = 0,
+    MODE_UNIQUE32 = 1,
+    MODE_UNIQUE64 = 2,
+    MODE_ECSCAN = 3,
+    MODE_SEG_UNIQUE32 = 4,
+    MODE_SEG_UNIQUE64 = 5,
+    MODE_SEG_SCAN = 6
+};
+
 // Entry points shared between translation units.
 int tc_fill_u32(tc_ctx* ctx, uint32_t* ptr, uint32_t value, size_t count, hipStream_t s);
-int tc_launch_scan_tiles(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0,
-                         void* out1, uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys,
-                         hipStream_t s);
+int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0, void* out1,
+                   uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, const uint32_t* seg_off,
+                   uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s);
 int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block);
 
 // ---------------------------------------------------------------- device side

@@ -23,3 +23,4 @@ from .api import (  # noqa: F401
     radix_sort,
     unique,
 )
+from .batch import Batch  # noqa: F401,E402

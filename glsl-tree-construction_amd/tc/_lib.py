@@ -67,6 +67,31 @@ class TcTree(C.Structure):
     ]
 
 
+CLOUD_INFO_DTYPE = np.dtype(
+    [("point_offset", "<u4"), ("n_points", "<u4"), ("unique_offset", "<u4"), ("n_unique", "<u4"),
+     ("node_offset", "<u4"), ("n_nodes", "<u4")]
+)
+
+
+class TcBatchView(C.Structure):
+    _fields_ = [
+        ("code_bits", C.c_int),
+        ("n_clouds", C.c_uint32),
+        ("n_points", C.c_uint32),
+        ("sorted", C.c_void_p),
+        ("unique", C.c_void_p),
+        ("prefix_n", C.c_void_p),
+        ("has_leaf_left", C.c_void_p),
+        ("has_leaf_right", C.c_void_p),
+        ("left_child", C.c_void_p),
+        ("parent", C.c_void_p),
+        ("edge_count", C.c_void_p),
+        ("offsets", C.c_void_p),
+        ("nodes", C.c_void_p),
+        ("node_capacity", C.c_uint64),
+    ]
+
+
 P = C.c_void_p
 I = C.c_int
 U32 = C.c_uint32
@@ -105,6 +130,8 @@ SIGNATURES = {
     "tc_unique_u64": (I, [P, P, P, U32, P, S]),
     "tc_build_radix_tree_u32": (I, [P, P, U32, P, P, P, P, P, P, S]),
     "tc_build_radix_tree_u64": (I, [P, P, U32, P, P, P, P, P, P, S]),
+    "tc_build_radix_tree_ec_u32": (I, [P, P, U32, P, P, P, P, P, P, P, S]),
+    "tc_build_radix_tree_ec_u64": (I, [P, P, U32, P, P, P, P, P, P, P, S]),
     "tc_edge_count": (I, [P, P, P, P, U32, P, S]),
     "tc_exclusive_scan_u32": (I, [P, P, P, U32, P, S]),
     "tc_edge_count_scan": (I, [P, P, P, P, P, U32, P, S]),
@@ -123,6 +150,14 @@ SIGNATURES = {
     "tc_host_edge_count": (I, [P, P, P, P, U32, U32]),
     "tc_host_partial_sum": (I, [P, P, U32, U32, P]),
     "tc_host_build_octree": (I, [P, P, P, P, P, P, P, F, F, C.c_int32, P, P, P, U32]),
+    "tc_batch_create": (I, [P, U32, U32, I, C.POINTER(C.c_void_p)]),
+    "tc_batch_destroy": (I, [P]),
+    "tc_batch_set": (I, [P, U32, P, P, P, S]),
+    "tc_batch_build": (I, [P, P, I, S]),
+    "tc_batch_capture": (I, [P, P, I, S]),
+    "tc_batch_replay": (I, [P, S]),
+    "tc_batch_read_info": (I, [P, P, S]),
+    "tc_batch_arrays": (I, [P, P]),
     "tc_gen_uniform": (None, [P, U64, U64, F]),
     "tc_gen_gaussian_mixture": (None, [P, U64, U64, U32, F, F]),
 }

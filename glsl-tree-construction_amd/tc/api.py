@@ -222,7 +222,7 @@ class Tree:
         res["left_child"] = dl(self.t.left_child, nb, np.int32)
         res["parent"] = dl(self.t.parent, nb, np.int32)
         res["edge_count"] = dl(self.t.edge_count, nb, np.uint32)
-        res["offsets"] = dl(self.t.offsets, nb + 1 if nb else 0, np.uint32)
+        res["offsets"] = dl(self.t.offsets, nb + 1, np.uint32) if n else np.zeros(1, np.uint32)
         raw = dl(self.t.nodes, no * 64, np.uint8)
         res["nodes"] = raw.view(OCT_NODE_DTYPE)
         return res

@@ -83,7 +83,7 @@ int tc_get_sort_params(tc_ctx* ctx, int* block, int* keys_per_thread);
  * enabled, every kernel launch is bracketed by two HIP events on its own
  * stream; collect synchronises on them and accumulates per-kernel totals.
  * Kernel ids: 0 morton, 1 histogram, 2 binning, 3 unique, 4 scan,
- * 5 radix_tree, 6 edge_count, 7 edge_count_scan, 8 octree_emit,
+ * 5 radix_tree, 6 edge_count, 7 edge_count_scan, 8 octree,
  * 9 octree_link, 10 fill, 11 decode, 12 segmented. */
 int tc_profile_enable(tc_ctx* ctx, int on);
 int tc_profile_collect(tc_ctx* ctx);
@@ -160,6 +160,19 @@ int tc_build_radix_tree_u64(tc_ctx* ctx, const uint64_t* codes, uint32_t n_keys,
                             uint8_t* has_leaf_right, int32_t* left_child, int32_t* parent,
                             tc_stream stream);
 
+/* Radix tree and edge counts in one pass (countEdges, shader/edge_count.comp:30-41,
+ * fused): each node writes the counts of its internal children, node 0 writes
+ * the root's 1; edge_count holds n_keys-1 entries.  Same arrays as
+ * tc_build_radix_tree_* followed by tc_edge_count. */
+int tc_build_radix_tree_ec_u32(tc_ctx* ctx, const uint32_t* codes, uint32_t n_keys,
+                               const uint32_t* d_n_keys, uint8_t* prefix_n, uint8_t* has_leaf_left,
+                               uint8_t* has_leaf_right, int32_t* left_child, int32_t* parent,
+                               uint32_t* edge_count, tc_stream stream);
+int tc_build_radix_tree_ec_u64(tc_ctx* ctx, const uint64_t* codes, uint32_t n_keys,
+                               const uint32_t* d_n_keys, uint8_t* prefix_n, uint8_t* has_leaf_left,
+                               uint8_t* has_leaf_right, int32_t* left_child, int32_t* parent,
+                               uint32_t* edge_count, tc_stream stream);
+
 /* ---- Edge count: countEdges, shader/edge_count.comp:30-41; host wrapper
  *      edge_count(), src/edge_count.rs:41-47.  n_brt = n_keys-1 entries;
  *      entry 0 (the root) is 1. */
@@ -226,6 +239,52 @@ int tc_build_tree(tc_ctx* ctx, const float* pts_xyzw, uint32_t n, float min_coor
                   int link_leaves, tc_tree* tree, tc_stream stream);
 /* Synchronise the stream and copy the device counts into the tree struct. */
 int tc_tree_read_counts(tc_ctx* ctx, tc_tree* tree, tc_stream stream);
+
+/* ---- Batched builds (no reference counterpart: the reference builds one cloud
+ *      per call, src/main.rs:60-170).  Many clouds of uneven sizes, concatenated
+ *      in one point array, are built with one persistent segmented launch per
+ *      stage; each cloud's outputs equal a plain tc_build_tree of its points.
+ *      Every kernel reads the batch shape from device memory, so a captured
+ *      graph (tc_batch_capture) replays for any batch within the capacity after
+ *      tc_batch_set uploads the new plan. */
+typedef struct tc_batch tc_batch;
+
+typedef struct {
+    uint32_t point_offset, n_points;   /* cloud's points in the concatenated input / sorted codes */
+    uint32_t unique_offset, n_unique;  /* cloud's unique codes and radix-node slots (n_unique - 1 nodes) */
+    uint32_t node_offset, n_nodes;     /* cloud's octree nodes; stored links are cloud-local */
+} tc_cloud_info;
+
+typedef struct {
+    int code_bits;
+    uint32_t n_clouds, n_points;
+    void* sorted;            /* [n_points] codes, sorted within each cloud */
+    void* unique;            /* unique codes, cloud c at unique_offset */
+    uint8_t* prefix_n;       /* radix arrays, cloud c at unique_offset (cloud-local indices) */
+    uint8_t* has_leaf_left;
+    uint8_t* has_leaf_right;
+    int32_t* left_child;
+    int32_t* parent;
+    uint32_t* edge_count;    /* cloud c at unique_offset */
+    uint32_t* offsets;       /* global node offsets; cloud-local = offsets - node_offset */
+    tc_oct_node* nodes;      /* cloud c at node_offset */
+    uint64_t node_capacity;
+} tc_batch_view;
+
+int tc_batch_create(tc_ctx* ctx, uint32_t max_clouds, uint32_t max_points, int code_bits, tc_batch** out);
+int tc_batch_destroy(tc_batch* batch);
+/* Describe the next batch: per-cloud point counts (and min/range, NULL for
+ * 0 / 1024); uploads the plan asynchronously on the stream. */
+int tc_batch_set(tc_batch* batch, uint32_t n_clouds, const uint32_t* counts, const float* mins,
+                 const float* ranges, tc_stream stream);
+int tc_batch_build(tc_batch* batch, const float* pts_xyzw, int link_leaves, tc_stream stream);
+/* Capture the whole batched pipeline (for pts_xyzw) as a HIP graph; replay it
+ * after each tc_batch_set.  The stream must not be the legacy NULL stream. */
+int tc_batch_capture(tc_batch* batch, const float* pts_xyzw, int link_leaves, tc_stream stream);
+int tc_batch_replay(tc_batch* batch, tc_stream stream);
+/* Synchronise the stream and read per-cloud offsets and counts. */
+int tc_batch_read_info(tc_batch* batch, tc_cloud_info* info, tc_stream stream);
+int tc_batch_arrays(tc_batch* batch, tc_batch_view* view);
 
 /* ---- Host-array mirrors of the reference's per-stage functions (blocking).
  * Argument order and meaning follow the Rust functions cited; sizes are

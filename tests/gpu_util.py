@@ -66,7 +66,7 @@ def run_stages(ctx, points: np.ndarray, min_coord: float, range_: float, code_bi
     out["edge_count"] = host(ec, np.uint32, nb)
     offs = empty(nb + 1, np.uint32)
     ctx.exclusive_scan(ec, offs, n=nb)
-    out["offsets"] = host(offs, np.uint32, nb + 1) if nb else np.zeros(0, np.uint32)
+    out["offsets"] = host(offs, np.uint32, nb + 1)
     total = int(out["offsets"][-1]) if nb else 0
     nodes = torch.zeros(max(total, 1) * 16, dtype=torch.int32, device="cuda")
     ctx.build_octree(nodes, total, offs, ec, uniq, pn, par, left, hll, hlr, nb, min_coord, range_, link_leaves)
