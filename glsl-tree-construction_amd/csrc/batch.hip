@@ -182,29 +182,25 @@ template <typename K, int BLOCK, int KPT>
 __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(Plan p, const K* __restrict__ sk, K* __restrict__ dk,
                                                         uint32_t shift, uint32_t pass, const uint32_t* __restrict__ seg_hist,
                                                         uint32_t* __restrict__ lb_cur, uint32_t* __restrict__ lb_next,
-                                                        uint32_t* __restrict__ tile_ctr) {
+                                                        uint32_t spin_limit) {
     using Sh = OnesweepShared<K, BLOCK, KPT>;
     constexpr int DIG = (int)sizeof(K);
     __shared__ Sh sm;
     const uint32_t C = p.desc[0], T = p.desc[1];
-    while (true) {
-        if (threadIdx.x == 0) {
-            const uint32_t t = atomicAdd(tile_ctr, 1u);
-            sm.tile = t;
-            sm.aux = t < T ? seg_find(p.first_tile, C, t) : 0u;
-        }
-        __syncthreads();
-        const uint32_t tile = sm.tile;
-        if (tile >= T) break;
-        const uint32_t c = sm.aux;
-        const uint32_t first = p.first_tile[c];
-        const uint32_t base = p.pt_off[c] + (tile - first) * Sh::TILE;
-        const uint32_t end = p.pt_off[c + 1];
-        const uint32_t cnt = end - base < Sh::TILE ? end - base : Sh::TILE;
-        onesweep_tile<K, false, BLOCK, KPT>(sm, sk, dk, nullptr, nullptr, base, cnt, shift, tile, first,
-                                            seg_hist + ((size_t)c * DIG + pass) * 256, p.pt_off[c], lb_cur, lb_next);
-        __syncthreads();
-    }
+    // Grid = tile capacity, tile = workgroup id (see k_onesweep).
+    const uint32_t tile = blockIdx.x;
+    if (tile >= T) return;
+    if (threadIdx.x == 0) sm.aux = seg_find(p.first_tile, C, tile);
+    __syncthreads();
+    const uint32_t c = sm.aux;
+    const uint32_t first = p.first_tile[c];
+    const uint32_t seg_base = p.pt_off[c];
+    const uint32_t base = seg_base + (tile - first) * Sh::TILE;
+    const uint32_t end = p.pt_off[c + 1];
+    const uint32_t cnt = end - base < Sh::TILE ? end - base : Sh::TILE;
+    onesweep_tile<K, false, BLOCK, KPT>(sm, sk, dk, nullptr, nullptr, base, cnt, shift, tile, first, seg_base,
+                                        seg_hist + ((size_t)c * DIG + pass) * 256, seg_base, lb_cur, lb_next,
+                                        spin_limit);
 }
 
 // Fix-ups after a segmented scan: empty segments get the base of the next
@@ -278,15 +274,13 @@ void launch_seg_pass(tc_batch* b, hipStream_t s, uint32_t grid, const K* src, K*
     uint32_t* lb0 = b->lb + (size_t)(pass & 1) * b->cap_tiles * 256;
     uint32_t* lb1 = b->lb + (size_t)((pass + 1) & 1) * b->cap_tiles * 256;
     hipLaunchKernelGGL((k_seg_onesweep<K, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, p, src, dst, 8u * pass, pass,
-                       b->seg_hist, lb0, (pass + 1 < sizeof(K)) ? lb1 : nullptr, b->ctrs + pass);
+                       b->seg_hist, lb0, (pass + 1 < sizeof(K)) ? lb1 : nullptr, b->ctx->spin_limit);
 }
 
 template <typename K>
 int seg_pass(tc_batch* b, hipStream_t s, const K* src, K* dst, uint32_t pass) {
-    // Persistent grid: enough workgroups to fill the GPU, never more than tiles.
-    uint32_t grid = (uint32_t)b->ctx->num_cus * (b->block == 256 ? 4 : 2);
-    if (grid > b->cap_tiles) grid = b->cap_tiles;
-    if (grid == 0) grid = 1;
+    // One workgroup per tile of capacity (a replayed graph keeps this grid).
+    const uint32_t grid = b->cap_tiles ? b->cap_tiles : 1;
 #define TC_SEG_CASE(B, KP)                                            \
     if (b->block == B && b->kpt == KP) {                              \
         launch_seg_pass<K, B, KP>(b, s, grid, src, dst, pass);        \

@@ -8,17 +8,19 @@
 // shader/edge_count.comp:30-41.  Published algorithm: Merrill & Garland,
 // "Single-pass Parallel Prefix Scan with Decoupled Look-back" (2016).
 //
-// gfx950 design: persistent workgroups claim 4096-element tiles from an atomic
-// counter (so every waited-on tile is owned by a running workgroup).  A tile is
+// gfx950 design: one 4096-element tile per workgroup, tile = workgroup id (a
+// claim counter would serialise at about 11 ns per claim on MI355X).  A tile is
 // loaded with 16 B vector loads (each wave instruction reads 1 KiB contiguous)
 // into LDS and read back so each thread owns 16 consecutive elements; a block
 // scan gives tile-local offsets; wave 0 publishes the tile aggregate and looks
-// back over 64 predecessors at a time.  Results go back through LDS so the
-// stores are contiguous too (unique compacts the tile's keys in LDS first).
-// Tile status words are u64 [value:32 | flag:2 | epoch:30]; the epoch is a
-// device-side launch sequence number, so status words never need clearing
-// (also under graph replay).  The last workgroup to exit resets the tile
-// counter and advances the sequence number.
+// back over 64 predecessors per round trip.  The wait is bounded: a workgroup
+// that has polled spin_limit times without an answer reduces the elements
+// before its tile itself (exact, slower), so progress never depends on the
+// order in which workgroups start.  Results go back through LDS so stores are
+// contiguous too (unique compacts the tile's keys in LDS first).
+// Tile status words are u64 [value:32 | flag:2 | epoch:30]; a one-lane kernel
+// advances the device-side epoch before each scan, so status words never need
+// clearing (also under graph replay).
 //
 // Segmented modes run one scan over many concatenated clouds: an element that
 // starts a cloud always counts as a new unique key, and the exclusive prefix at
@@ -32,10 +34,10 @@ constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_IPT = 16;
 constexpr uint32_t SCAN_TILE = SCAN_BLOCK * SCAN_IPT;
 
-// Scan control words (in ctx->small): tile counter, exit counter, launch sequence.
+// Scan control words (in ctx->small).
 struct ScanCtl {
-    uint32_t tile_ctr;
-    uint32_t exit_ctr;
+    uint32_t reserved0;
+    uint32_t reserved1;
     uint32_t seq;
 };
 
@@ -53,6 +55,7 @@ struct ScanArgs {
     uint32_t* seg_out;        // segmented: [n_seg] exclusive prefix at each segment start
     uint64_t* status;
     ScanCtl* ctl;
+    uint32_t spin_limit;
 };
 
 constexpr bool is_seg(int m) { return m == MODE_SEG_UNIQUE32 || m == MODE_SEG_UNIQUE64 || m == MODE_SEG_SCAN; }
@@ -75,11 +78,14 @@ TC_DEV uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Look-back by wave 0; returns the exclusive prefix of `tile`.
-TC_DEV uint32_t wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch) {
+// Look-back by wave 0; returns the exclusive prefix of `tile`, or sets
+// *gave_up after spin_limit empty polls.
+TC_DEV uint32_t wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch, uint32_t spin_limit,
+                              bool* gave_up) {
     const uint32_t lane = lane_id();
-    uint32_t excl = 0;
+    uint32_t excl = 0, spins = 0;
     int64_t top = (int64_t)tile - 1;
+    *gave_up = false;
     while (true) {
         const int64_t idx = top - (int64_t)lane;
         uint32_t f, v;
@@ -99,6 +105,10 @@ TC_DEV uint32_t wave_lookback(const uint64_t* status, uint32_t tile, uint32_t ep
             // Lanes up to and including the nearest inclusive predecessor.
             upto = incl_mask ? ((incl_mask & (~incl_mask + 1)) << 1) - 1 : ~0ull;
             if ((nr & upto) == 0) break;
+            if (++spins > spin_limit) {
+                *gave_up = true;
+                return 0u;
+            }
             __builtin_amdgcn_s_sleep(1);
         }
         const bool take = (upto >> lane) & 1ull;
@@ -120,33 +130,63 @@ TC_DEV uint32_t find_seg(const uint32_t* off, uint32_t nseg, uint32_t i) {
     return lo;
 }
 
-// Coalesced tile load into LDS (tile-relative positions), then per-thread
-// blocked read-back is done by the caller.
+// Scan value of element i (the fall-back path's reduction over [0, base)).
+template <int MODE>
+TC_DEV uint32_t elem_value(const ScanArgs& a, uint32_t n, uint32_t i, uint32_t nseg, uint32_t& seg,
+                           uint32_t& seg_start, uint32_t& seg_next) {
+    using T = ItemT<MODE>;
+    if (is_plain_values(MODE)) return static_cast<const uint32_t*>(a.in0)[i];
+    if (is_unique(MODE)) {
+        const T* in = static_cast<const T*>(a.in0);
+        bool starts = (i == 0);
+        if (is_seg(MODE)) {
+            while (i >= seg_next && seg + 1 < nseg) {
+                ++seg;
+                seg_start = a.seg_off[seg];
+                seg_next = a.seg_off[seg + 1];
+            }
+            starts = starts || (i == seg_start);
+        }
+        return (starts || in[i] != in[i - 1]) ? 1u : 0u;
+    }
+    // MODE_ECSCAN
+    const uint8_t* pn = static_cast<const uint8_t*>(a.in0);
+    uint32_t p = (uint32_t)static_cast<const int32_t*>(a.in1)[i];
+    p = p < n ? p : 0u;
+    return i == 0 ? 1u : (uint32_t)(pn[i] / 3 - pn[p] / 3);
+}
+
+// Coalesced tile load into LDS (tile-relative positions).
 template <typename T>
-TC_DEV void load_tile(const T* __restrict__ in, uint32_t base, uint32_t cnt, T* s) {
+TC_DEV void load_tile(const T* __restrict__ in, uint32_t base, uint32_t cnt, T* s, bool aligned) {
     constexpr int VEC = 16 / sizeof(T);
-    if (cnt == SCAN_TILE) {
+    if (aligned && cnt == SCAN_TILE) {
         using V = typename std::conditional<sizeof(T) == 4, uint4, ulonglong2>::type;
         const V* vin = reinterpret_cast<const V*>(in + base);
         V* vs = reinterpret_cast<V*>(s);
 #pragma unroll
-        for (int r = 0; r < (int)(SCAN_TILE / VEC / SCAN_BLOCK); ++r) vs[r * SCAN_BLOCK + threadIdx.x] = vin[r * SCAN_BLOCK + threadIdx.x];
+        for (int r = 0; r < (int)(SCAN_TILE / VEC / SCAN_BLOCK); ++r)
+            vs[r * SCAN_BLOCK + threadIdx.x] = vin[r * SCAN_BLOCK + threadIdx.x];
     } else {
         for (uint32_t k = threadIdx.x; k < cnt; k += SCAN_BLOCK) s[k] = in[base + k];
     }
+}
+
+__global__ void k_scan_prep(ScanCtl* ctl) {
+    // Advance the epoch tag: status words of earlier scans read as not ready.
+    ctl->seq = (ctl->seq + 1u) & 0x3FFFFFFFu;
 }
 
 template <int MODE>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     using T = ItemT<MODE>;
     __shared__ uint32_t s_tmp[SCAN_BLOCK / 64 + 2];
-    __shared__ uint32_t s_tile;
     __shared__ uint32_t s_excl;
+    __shared__ uint32_t s_fallback;
     __shared__ __attribute__((aligned(16))) T s_items[SCAN_TILE];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6;
-    const uint32_t epoch = ld_acquire_agent(&a.ctl->seq) & 0x3FFFFFFFu;
 
     uint32_t n = a.n_cap;
     uint32_t nseg = 0;
@@ -161,172 +201,172 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         n = m < a.n_cap ? m : a.n_cap;
     }
     const uint32_t num_tiles = n ? ceil_div_u32(n, SCAN_TILE) : 1u;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= num_tiles) return;  // capacity grids: workgroups past the data leave at once
+    const uint32_t epoch = a.ctl->seq & 0x3FFFFFFFu;
     // Unaligned inputs (tile starts not 16 B aligned) take the element path.
     const bool aligned = (reinterpret_cast<uintptr_t>(a.in0) & 15u) == 0;
 
-    while (true) {
-        if (tid == 0) s_tile = atomicAdd(&a.ctl->tile_ctr, 1u);
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= num_tiles) break;
+    const uint32_t base = tile * SCAN_TILE;
+    const uint32_t cnt = n > base ? (n - base < SCAN_TILE ? n - base : SCAN_TILE) : 0u;
+    const uint32_t r0 = tid * SCAN_IPT;  // tile-relative index of this thread's first element
+    const uint32_t i0 = base + r0;
+    uint32_t val[SCAN_IPT];
+    T item[is_unique(MODE) ? SCAN_IPT : 1];
+    uint32_t start_mask = 0;  // bit j: element i0 + j starts a segment
+    uint32_t seg_first = 0;
 
-        const uint32_t base = tile * SCAN_TILE;
-        const uint32_t cnt = n > base ? (n - base < SCAN_TILE ? n - base : SCAN_TILE) : 0u;
-        const uint32_t r0 = tid * SCAN_IPT;  // tile-relative index of this thread's first element
-        const uint32_t i0 = base + r0;
-        uint32_t val[SCAN_IPT];
-        T item[is_unique(MODE) ? SCAN_IPT : 1];
-        uint32_t start_mask = 0;  // bit j: element i0 + j starts a segment
-        uint32_t seg_first = 0;
-
-        if (MODE != MODE_ECSCAN) {
-            if (aligned) load_tile<T>(static_cast<const T*>(a.in0), base, cnt, s_items);
-            else
-                for (uint32_t k = tid; k < cnt; k += SCAN_BLOCK) s_items[k] = static_cast<const T*>(a.in0)[base + k];
-            __syncthreads();
-        }
-        if (is_seg(MODE) && nseg > 0 && i0 < n) {
-            uint32_t seg = find_seg(a.seg_off, nseg, i0);
-            seg_first = seg;
-            uint32_t seg_start = a.seg_off[seg], seg_next = a.seg_off[seg + 1];
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) {
-                const uint32_t i = i0 + j;
-                if (i >= n) break;
-                while (i >= seg_next && seg + 1 < nseg) {
-                    ++seg;
-                    seg_start = a.seg_off[seg];
-                    seg_next = a.seg_off[seg + 1];
-                }
-                if (i == seg_start) start_mask |= 1u << j;
-            }
-        }
-
-        if (is_plain_values(MODE)) {
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) val[j] = (r0 + j < cnt) ? (uint32_t)s_items[r0 + j] : 0u;
-        } else if (is_unique(MODE)) {
-            T prev = (T)0;
-            if (r0 < cnt) prev = r0 > 0 ? s_items[r0 - 1] : (base > 0 ? static_cast<const T*>(a.in0)[base - 1] : (T)0);
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) {
-                const uint32_t r = r0 + j;
-                const T k = r < cnt ? s_items[r] : (T)0;
-                const bool starts = (base + r == 0) || ((start_mask >> j) & 1u);
-                item[j] = k;
-                val[j] = (r < cnt && (starts || k != prev)) ? 1u : 0u;
-                prev = k;
-            }
-        } else {  // MODE_ECSCAN: edge count of radix node i, root entry 1
-            const uint8_t* pn = static_cast<const uint8_t*>(a.in0);
-            const int32_t* par = static_cast<const int32_t*>(a.in1);
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) {
-                const uint32_t g = i0 + j;
-                uint32_t e = 0;
-                if (g < n) {
-                    uint32_t p = (uint32_t)par[g];
-                    p = p < n ? p : 0u;  // parent indices of a valid tree are < n
-                    e = (g == 0) ? 1u : (uint32_t)(pn[g] / 3 - pn[p] / 3);
-                }
-                val[j] = e;
-            }
-        }
-
-        uint32_t tsum = 0;
-#pragma unroll
-        for (int j = 0; j < SCAN_IPT; ++j) tsum += val[j];
-        uint32_t block_total;
-        const uint32_t texcl = block_exclusive_scan<SCAN_BLOCK>(tsum, s_tmp, &block_total);
-
-        if (wave == 0) {
-            const uint32_t lane = lane_id();
-            if (tile == 0) {
-                if (lane == 0) {
-                    st_agent64(&a.status[0], st_word(2u, epoch, block_total));
-                    s_excl = 0;
-                }
-            } else {
-                if (lane == 0) st_agent64(&a.status[tile], st_word(1u, epoch, block_total));
-                const uint32_t ex = wave_lookback(a.status, tile, epoch);
-                if (lane == 0) {
-                    st_agent64(&a.status[tile], st_word(2u, epoch, ex + block_total));
-                    s_excl = ex;
-                }
-            }
-        }
-        __syncthreads();
-        const uint32_t tile_excl = s_excl;
-
-        // Segment bases: the exclusive prefix at each segment's first element.
-        if (is_seg(MODE) && start_mask) {
-            uint32_t seg = seg_first;
-            uint32_t run = tile_excl + texcl;
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) {
-                if ((start_mask >> j) & 1u) {
-                    // the last segment starting here (earlier equal offsets are empty clouds)
-                    while (seg + 1 < nseg && a.seg_off[seg + 1] <= i0 + j) ++seg;
-                    a.seg_out[seg] = run;
-                }
-                run += val[j];
-            }
-        }
-
-        if (is_plain_values(MODE)) {
-            // Exclusive values through LDS, then contiguous stores.
-            uint32_t run = tile_excl + texcl;
-            uint32_t* s_out = reinterpret_cast<uint32_t*>(s_items);
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) {
-                s_out[r0 + j] = run;
-                run += val[j];
-            }
-            __syncthreads();
-            uint32_t* out = static_cast<uint32_t*>(a.out0);
-            for (uint32_t k = tid; k < cnt; k += SCAN_BLOCK) out[base + k] = s_out[k];
-        } else if (is_unique(MODE)) {
-            // Compact the tile's kept keys in LDS, then store them contiguously.
-            uint32_t run = texcl;
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) {
-                if (val[j]) s_items[run] = item[j];
-                run += val[j];
-            }
-            __syncthreads();
-            T* out = static_cast<T*>(a.out0);
-            for (uint32_t k = tid; k < block_total; k += SCAN_BLOCK) out[tile_excl + k] = s_items[k];
-        } else {
-            uint32_t run = tile_excl + texcl;
-            uint32_t* ec = static_cast<uint32_t*>(a.out0);
-            uint32_t* off = static_cast<uint32_t*>(a.out1);
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) {
-                if (i0 + j < n) {
-                    ec[i0 + j] = val[j];
-                    off[i0 + j] = run;
-                }
-                run += val[j];
-            }
-        }
-        if (tile == num_tiles - 1 && tid == SCAN_BLOCK - 1) {
-            const uint32_t total = tile_excl + block_total;
-            if (is_plain_values(MODE)) static_cast<uint32_t*>(a.out0)[n] = total;
-            if (MODE == MODE_ECSCAN) static_cast<uint32_t*>(a.out1)[n] = total;
-            if (a.count_out) *a.count_out = total;
-        }
+    if (MODE != MODE_ECSCAN) {
+        load_tile<T>(static_cast<const T*>(a.in0), base, cnt, s_items, aligned);
         __syncthreads();
     }
-
-    // Exit protocol: the last workgroup out resets the counter for the next launch.
-    if (tid == 0) {
-        __threadfence();
-        const uint32_t t = atomicAdd(&a.ctl->exit_ctr, 1u);
-        if (t == gridDim.x - 1) {
-            st_agent(&a.ctl->tile_ctr, 0u);
-            st_agent(&a.ctl->exit_ctr, 0u);
-            st_agent(&a.ctl->seq, (epoch + 1u) & 0x3FFFFFFFu);
+    if (is_seg(MODE) && nseg > 0 && i0 < n) {
+        uint32_t seg = find_seg(a.seg_off, nseg, i0);
+        seg_first = seg;
+        uint32_t seg_start = a.seg_off[seg], seg_next = a.seg_off[seg + 1];
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) {
+            const uint32_t i = i0 + j;
+            if (i >= n) break;
+            while (i >= seg_next && seg + 1 < nseg) {
+                ++seg;
+                seg_start = a.seg_off[seg];
+                seg_next = a.seg_off[seg + 1];
+            }
+            if (i == seg_start) start_mask |= 1u << j;
         }
+    }
+
+    if (is_plain_values(MODE)) {
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) val[j] = (r0 + j < cnt) ? (uint32_t)s_items[r0 + j] : 0u;
+    } else if (is_unique(MODE)) {
+        T prev = (T)0;
+        if (r0 < cnt) prev = r0 > 0 ? s_items[r0 - 1] : (base > 0 ? static_cast<const T*>(a.in0)[base - 1] : (T)0);
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) {
+            const uint32_t r = r0 + j;
+            const T k = r < cnt ? s_items[r] : (T)0;
+            const bool starts = (base + r == 0) || ((start_mask >> j) & 1u);
+            item[j] = k;
+            val[j] = (r < cnt && (starts || k != prev)) ? 1u : 0u;
+            prev = k;
+        }
+    } else {  // MODE_ECSCAN: edge count of radix node i, root entry 1
+        uint32_t z0 = 0, z1 = 0, z2 = 0;
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) val[j] = (i0 + j < n) ? elem_value<MODE>(a, n, i0 + j, 0, z0, z1, z2) : 0u;
+    }
+
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; ++j) tsum += val[j];
+    uint32_t block_total;
+    const uint32_t texcl = block_exclusive_scan<SCAN_BLOCK>(tsum, s_tmp, &block_total);
+
+    if (tid == 0) s_fallback = (a.spin_limit == 0u && tile > 0) ? 1u : 0u;  // 0: always reduce directly
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t lane = lane_id();
+        if (tile == 0) {
+            if (lane == 0) {
+                st_agent64(&a.status[0], st_word(2u, epoch, block_total));
+                s_excl = 0;
+            }
+        } else {
+            if (lane == 0) st_agent64(&a.status[tile], st_word(1u, epoch, block_total));
+            if (!s_fallback) {
+                bool gave_up = false;
+                const uint32_t ex = wave_lookback(a.status, tile, epoch, a.spin_limit, &gave_up);
+                if (lane == 0) {
+                    if (gave_up) s_fallback = 1u;
+                    else s_excl = ex;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (s_fallback) {
+        // Reduce [0, base) directly: each thread takes a contiguous chunk.
+        const uint32_t chunk = ceil_div_u32(base, SCAN_BLOCK);
+        const uint32_t lo = tid * chunk, hi = lo + chunk < base ? lo + chunk : base;
+        uint32_t sum = 0, seg = 0, seg_start = 0, seg_next = 0xFFFFFFFFu;
+        if (is_seg(MODE) && nseg > 0 && lo < hi) {
+            seg = find_seg(a.seg_off, nseg, lo);
+            seg_start = a.seg_off[seg];
+            seg_next = a.seg_off[seg + 1];
+        }
+        for (uint32_t i = lo; i < hi; ++i) sum += elem_value<MODE>(a, n, i, nseg, seg, seg_start, seg_next);
+        uint32_t before;
+        block_exclusive_scan<SCAN_BLOCK>(sum, s_tmp, &before);
+        if (tid == 0) {
+            s_excl = before;
+            st_agent64(&a.status[tile], st_word(2u, epoch, before + block_total));
+        }
+        __syncthreads();
+    } else if (wave == 0 && tile > 0 && lane_id() == 0) {
+        st_agent64(&a.status[tile], st_word(2u, epoch, s_excl + block_total));
+    }
+    const uint32_t tile_excl = s_excl;
+
+    // Segment bases: the exclusive prefix at each segment's first element.
+    if (is_seg(MODE) && start_mask) {
+        uint32_t seg = seg_first;
+        uint32_t run = tile_excl + texcl;
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) {
+            if ((start_mask >> j) & 1u) {
+                // the last segment starting here (earlier equal offsets are empty clouds)
+                while (seg + 1 < nseg && a.seg_off[seg + 1] <= i0 + j) ++seg;
+                a.seg_out[seg] = run;
+            }
+            run += val[j];
+        }
+    }
+
+    if (is_plain_values(MODE)) {
+        // Exclusive values through LDS, then contiguous stores.
+        uint32_t run = tile_excl + texcl;
+        uint32_t* s_out = reinterpret_cast<uint32_t*>(s_items);
+        __syncthreads();  // everyone has read its items
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) {
+            s_out[r0 + j] = run;
+            run += val[j];
+        }
+        __syncthreads();
+        uint32_t* out = static_cast<uint32_t*>(a.out0);
+        for (uint32_t k = tid; k < cnt; k += SCAN_BLOCK) out[base + k] = s_out[k];
+    } else if (is_unique(MODE)) {
+        // Compact the tile's kept keys in LDS, then store them contiguously.
+        uint32_t run = texcl;
+        __syncthreads();  // everyone has read its items
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) {
+            if (val[j]) s_items[run] = item[j];
+            run += val[j];
+        }
+        __syncthreads();
+        T* out = static_cast<T*>(a.out0);
+        for (uint32_t k = tid; k < block_total; k += SCAN_BLOCK) out[tile_excl + k] = s_items[k];
+    } else {
+        uint32_t run = tile_excl + texcl;
+        uint32_t* ec = static_cast<uint32_t*>(a.out0);
+        uint32_t* off = static_cast<uint32_t*>(a.out1);
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) {
+            if (i0 + j < n) {
+                ec[i0 + j] = val[j];
+                off[i0 + j] = run;
+            }
+            run += val[j];
+        }
+    }
+    if (tile == num_tiles - 1 && tid == SCAN_BLOCK - 1) {
+        const uint32_t total = tile_excl + block_total;
+        if (is_plain_values(MODE)) static_cast<uint32_t*>(a.out0)[n] = total;
+        if (MODE == MODE_ECSCAN) static_cast<uint32_t*>(a.out1)[n] = total;
+        if (a.count_out) *a.count_out = total;
     }
 }
 
@@ -352,19 +392,20 @@ int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void
     if (!ctx->scan_status) return TC_ERR_NOMEM;
     // Status words for up to 2^30 / 4096 tiles are preallocated with the context.
     if (tiles > (1u << 18)) return TC_ERR_TOO_LARGE;
+    ScanCtl* ctl = reinterpret_cast<ScanCtl*>(ctx->small + SMALL_SCAN_CTR);
     ScanArgs a{in0, in1, out0, out1, count_out, n, d_n_keys, seg_off, n_seg_cap, d_n_seg, seg_out,
-               ctx->scan_status, reinterpret_cast<ScanCtl*>(ctx->small + SMALL_SCAN_CTR)};
-    const uint32_t cap = (uint32_t)ctx->num_cus * 4;
-    const uint32_t grid = tiles < cap ? tiles : cap;
+               ctx->scan_status, ctl, ctx->spin_limit};
     ProfScope prof(ctx, s, is_seg(mode) ? K_SEG : mode == MODE_SCAN ? K_SCAN : mode == MODE_ECSCAN ? K_ECSCAN : K_UNIQUE);
+    hipLaunchKernelGGL(k_scan_prep, dim3(1), dim3(1), 0, s, ctl);
+    TC_LAUNCH_CHECK();
     switch (mode) {
-        case MODE_SCAN: launch<MODE_SCAN>(grid, s, a); break;
-        case MODE_UNIQUE32: launch<MODE_UNIQUE32>(grid, s, a); break;
-        case MODE_UNIQUE64: launch<MODE_UNIQUE64>(grid, s, a); break;
-        case MODE_ECSCAN: launch<MODE_ECSCAN>(grid, s, a); break;
-        case MODE_SEG_UNIQUE32: launch<MODE_SEG_UNIQUE32>(grid, s, a); break;
-        case MODE_SEG_UNIQUE64: launch<MODE_SEG_UNIQUE64>(grid, s, a); break;
-        case MODE_SEG_SCAN: launch<MODE_SEG_SCAN>(grid, s, a); break;
+        case MODE_SCAN: launch<MODE_SCAN>(tiles, s, a); break;
+        case MODE_UNIQUE32: launch<MODE_UNIQUE32>(tiles, s, a); break;
+        case MODE_UNIQUE64: launch<MODE_UNIQUE64>(tiles, s, a); break;
+        case MODE_ECSCAN: launch<MODE_ECSCAN>(tiles, s, a); break;
+        case MODE_SEG_UNIQUE32: launch<MODE_SEG_UNIQUE32>(tiles, s, a); break;
+        case MODE_SEG_UNIQUE64: launch<MODE_SEG_UNIQUE64>(tiles, s, a); break;
+        case MODE_SEG_SCAN: launch<MODE_SEG_SCAN>(tiles, s, a); break;
         default: return TC_ERR_INVALID;
     }
     TC_LAUNCH_CHECK();
@@ -402,4 +443,10 @@ extern "C" int tc_edge_count_scan(tc_ctx* ctx, const uint8_t* prefix_n, const in
     if (n_brt > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
     return tc_launch_scan(ctx, MODE_ECSCAN, prefix_n, parent, edge_count, offsets, nullptr, n_brt, d_n_keys,
                           nullptr, 0, nullptr, nullptr, as_stream(stream));
+}
+
+extern "C" int tc_set_lookback_spin_limit(tc_ctx* ctx, uint32_t polls) {
+    if (!ctx) return TC_ERR_INVALID;
+    ctx->spin_limit = polls;
+    return TC_OK;
 }

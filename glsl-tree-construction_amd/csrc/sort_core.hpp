@@ -11,15 +11,21 @@
 // Peer mask: lanes of this wave whose 8-bit digit equals mine, among `live`
 // lanes (8 ballots; the reference's 32-lane version is
 // shader/new_radix_sort.comp:147-152).  Must be called with all lanes active.
+//
+// Per bit b: rep = 0 or ~0 (my bit b, sign-extended), m = ballot of bit b;
+// rep ^ m has a 1 for every lane whose bit b differs from mine.  OR-ing those
+// over the 8 bits gives the lanes with a different digit (xor + or3 per half
+// word, about 40 VALU ops per key instead of select-and-and chains).
 TC_DEV uint64_t match_digit8(uint32_t d, uint64_t live) {
-    uint64_t peers = live;
+    uint32_t diff_lo = 0u, diff_hi = 0u;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+        const uint32_t rep = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        const uint64_t m = __ballot(rep != 0u);
+        diff_lo |= rep ^ (uint32_t)m;
+        diff_hi |= rep ^ (uint32_t)(m >> 32);
     }
-    return peers;
+    return live & ~(((uint64_t)diff_hi << 32) | diff_lo);
 }
 
 template <typename K>
@@ -35,8 +41,10 @@ struct OnesweepShared {
     uint32_t start[256];
     uint32_t gbase[256];
     uint32_t tmp[WAVES + 2];
+    uint32_t fb[256];  // fall-back digit counts
     uint32_t tile;
     uint32_t aux;
+    uint32_t fallback;
     alignas(16) unsigned char stage[TILE * sizeof(K)];
 };
 
@@ -47,12 +55,14 @@ struct OnesweepShared {
 // look-back walks down to `first`, the first tile of the segment, which
 // publishes its counts as inclusive.  lb_next (may be NULL) is the next pass's
 // look-back row of this tile, cleared here.
+// seg_base is the index of the segment's first key (the fall-back counts the
+// keys in [seg_base, base)); spin_limit bounds the look-back wait.
 template <typename K, bool PAIRS, int BLOCK, int KPT>
 TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict__ sk, K* __restrict__ dk,
                           const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
-                          uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first,
+                          uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first, uint32_t seg_base,
                           const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
-                          uint32_t* __restrict__ lb_next) {
+                          uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
     constexpr int WAVES = BLOCK / 64;
     constexpr uint32_t TILE = BLOCK * KPT;
     constexpr uint32_t WAVE_TILE = 64 * KPT;
@@ -140,24 +150,55 @@ TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict
         }
     }
 
-    // Decoupled look-back, one thread per digit, down to the segment's first tile.
-    if (tid < 256) {
-        uint32_t excl = 0;
-        if (tile != first) {
-            uint32_t k = tile - 1;
-            while (true) {
-                const uint32_t v = ld_acquire_agent(&lb_cur[k * 256 + tid]);
-                const uint32_t f = v & LB_FLAG_MASK;
-                if (f == LB_NOT_READY) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += v >> 2;
-                if (f == LB_INCLUSIVE) break;
-                --k;
+    // Decoupled look-back, one thread per digit, down to the segment's first tile
+    // (which always publishes inclusive counts, so the walk never passes it).
+    // Four predecessors are fetched per round trip: agent-scope loads bypass
+    // the XCD-local L2, so each round trip is long.  The wait is bounded: after
+    // spin_limit empty polls the whole workgroup counts the digits of the
+    // segment's earlier keys itself (exact, just slower), so progress never
+    // depends on the order in which the hardware starts workgroups.
+    if (tid == 0) sm.fallback = (spin_limit == 0u && tile != first) ? 1u : 0u;  // 0: always count directly
+    uint32_t excl = 0;
+    __syncthreads();
+    if (tid < 256 && tile != first && spin_limit != 0u) {
+        uint32_t k = tile - 1;
+        uint32_t spins = 0;
+        while (true) {
+            uint32_t v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = (k >= first + j) ? ld_acquire_agent(&lb_cur[(k - j) * 256 + tid]) : 0u;
+            bool found = false;
+            uint32_t used = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (found || used != (uint32_t)j) break;
+                const uint32_t f = v[j] & LB_FLAG_MASK;
+                if (f == LB_NOT_READY) break;
+                excl += v[j] >> 2;
+                if (f == LB_INCLUSIVE) found = true;
+                else ++used;
             }
-            st_agent(&lb_cur[tile * 256 + tid], LB_INCLUSIVE | ((excl + total_d) << 2));
+            if (found) break;
+            k -= used;
+            if (used < 4) {
+                if (++spins > spin_limit) {
+                    sm.fallback = 1u;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
         }
+    }
+    __syncthreads();
+    if (sm.fallback) {
+        for (uint32_t i = tid; i < 256; i += BLOCK) sm.fb[i] = 0u;
+        __syncthreads();
+        for (uint32_t i = seg_base + tid; i < base; i += BLOCK) atomicAdd(&sm.fb[digit_of<K>(sk[i], shift)], 1u);
+        __syncthreads();
+        if (tid < 256) excl = sm.fb[tid];
+    }
+    if (tid < 256) {
+        if (tile != first) st_agent(&lb_cur[tile * 256 + tid], LB_INCLUSIVE | ((excl + total_d) << 2));
         sm.gbase[tid] = prefix[tid] + add + excl - start;
     }
     __syncthreads();

@@ -70,8 +70,11 @@ struct tc_ctx {
     uint64_t prof_n[K_COUNT] = {};
     int num_cus = 256;
     // Sort launch parameters.
-    int sort_block = 256;
-    int sort_kpt = 16;
+    int sort_block = 512;
+    int sort_kpt = 20;
+    // Look-back wait bound (empty polls) before a workgroup computes its
+    // prefix itself; tests set it to 0 to exercise that path.
+    uint32_t spin_limit = 1u << 15;
     // Grow-only device scratch.
     void* scratch = nullptr;
     size_t scratch_bytes = 0;

@@ -107,6 +107,7 @@ SIGNATURES = {
     "tc_destroy": (I, [P]),
     "tc_reserve": (I, [P, U32]),
     "tc_set_sort_params": (I, [P, I, I]),
+    "tc_set_lookback_spin_limit": (I, [P, U32]),
     "tc_profile_enable": (I, [P, I]),
     "tc_profile_collect": (I, [P]),
     "tc_profile_reset": (I, [P]),

@@ -77,6 +77,11 @@ int tc_reserve(tc_ctx* ctx, uint32_t max_n);
  * keys per thread (8, 12, 16 or 20).  0 keeps the current value. */
 int tc_set_sort_params(tc_ctx* ctx, int block, int keys_per_thread);
 int tc_get_sort_params(tc_ctx* ctx, int* block, int* keys_per_thread);
+/* Bound on a workgroup's look-back wait (empty polls of a predecessor's
+ * status) in the sort and scan kernels; past it the workgroup computes its
+ * prefix from the input itself, so progress never depends on workgroup start
+ * order.  0 makes every workgroup take that path (used to test it). */
+int tc_set_lookback_spin_limit(tc_ctx* ctx, uint32_t polls);
 
 /* Per-launch event profiler (the reference times each stage with
  * Instant::now() around submit + fence, e.g. src/morton.rs:327-352).  When

@@ -128,3 +128,19 @@ def test_batch_rejects_over_capacity(gpu_ctx):
     with pytest.raises(tc.TcError):
         b.set(np.ones(5, np.uint32))
     b.close()
+
+
+def test_batch_lookback_fallback_path(gpu_ctx, oracle, tclib):
+    import tc
+
+    L = tc.lib()
+    clouds = make_clouds(tclib, [20_000, 3, 50_000, 1, 9000], 4242)
+    assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 0) == 0
+    try:
+        b = tc.Batch(gpu_ctx, 8, 100_000, 30)
+        run_batch(b, clouds)
+        for i, (g, pts) in enumerate(zip(b.download(), clouds)):
+            check_cloud(oracle, g, pts, 0.0, 1024.0, 30, f"cloud {i}")
+        b.close()
+    finally:
+        assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 1 << 15) == 0

@@ -252,3 +252,27 @@ def test_invalid_arguments_rejected(gpu_ctx):
     assert L.tc_morton30(h, ctypes.c_void_p(8), ctypes.c_void_p(16), 4, 0.0, 1.0, None) == tc._lib.TC_ERR_INVALID
     k = empty(16, np.uint32)
     assert L.tc_sort_keys_u32(h, k.data_ptr(), k.data_ptr(), (1 << 30) + 1, None) == tc._lib.TC_ERR_TOO_LARGE
+
+
+def test_lookback_fallback_path(gpu_ctx, oracle, tclib):
+    """With the look-back wait bound at 0 every workgroup computes its prefix
+    from the input itself; results must be identical to the look-back path."""
+    import tc
+
+    L = tc.lib()
+    pts = tclib.gen_gaussian_mixture(300_000, 77, 32, 6.0)
+    want = oracle_all(oracle, pts, 0.0, 1024.0, 30)
+    assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 0) == 0
+    try:
+        got = run_stages(gpu_ctx, pts, 0.0, 1024.0, 30)
+        assert_same(got, want)
+        r = np.random.default_rng(5)
+        k64 = r.integers(0, 1 << 63, size=70_000, dtype=np.uint64)
+        k64[::5] = k64[1]
+        vals = np.arange(len(k64), dtype=np.uint32)
+        k, v = dev(k64), dev(vals)
+        gpu_ctx.sort(k, empty(len(k64), np.uint64), v, empty(len(k64), np.uint32))
+        sk, sv = oracle.radix_sort(k64, vals)
+        assert np.array_equal(host(k, np.uint64), sk) and np.array_equal(host(v, np.uint32), sv)
+    finally:
+        assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 1 << 15) == 0
