@@ -1,16 +1,25 @@
-"""Benchmark: full plain tree build on one MI355X.
+"""Benchmark: full plain tree build on one MI355X, plus the sort-dominated and
+batched workloads of the same metric.
 
-Workload (default): 2^26 points drawn from a 64-centre Gaussian mixture
+Headline workload: 2^26 points drawn from a 64-centre Gaussian mixture
 (centres uniform in [0,1024)^3, sigma 8, clipped to the box), float32 xyzw,
-30-bit Morton codes.  A step is one complete build -- morton, histogram, 4
-OneSweep binning passes, unique, radix tree, fused edge count + scan, octree
-emit + link -- on points already resident in HBM, queued without host
-synchronisation.
+30-bit Morton codes.  A step is one complete build -- Morton codes with the
+sort histogram fused in, 4 OneSweep binning passes, unique, radix tree with
+fused edge counts, exclusive scan, octree -- on points already resident in
+HBM, queued without host synchronisation.  `value` = points / step time.
 
-Prints one JSON line (rank 0).  `roofline` is for the dominant kernel
-(measured with HIP events around each launch on its own stream, outside the
-timed loop); `cpu_baseline` times the single-threaded CPU port of the same
-pipeline (the oracle) on the same input, on this host.
+Also reported (same JSON line):
+  roofline      dominant kernel (the binning pass): algorithmic bytes / average
+                launch time, launch times from HIP events around each launch on
+                its own stream (outside the timed loop)
+  sort          standalone u32 key sort of the same codes (histogram + 4
+                passes) against the 36 B/key algorithmic traffic
+  sort_pairs_u64  2^27 63-bit keys with u32 payloads, 8 passes (200 B/pair)
+  batched       1024 clouds, sizes log-uniform in [2^10, 2^20], alternating
+                uniform and 16-cluster Gaussian (sigma 4): one segmented launch
+                per stage replayed as a HIP graph vs one plain build per cloud
+  cpu_baseline  the single-threaded C port of the pipeline (the oracle) on the
+                same input, on this host
 """
 import argparse
 import json
@@ -30,15 +39,40 @@ def algorithmic_bytes(n, u, o):
     algorithm needs, counting each array once)."""
     nb = max(u - 1, 0)
     return {
-        "morton": 20 * n,            # 16 B point in, 4 B code out
-        "histogram": 4 * n,          # one read of the codes
+        "morton": 20 * n,            # 16 B point in, 4 B code out (histogram fused, no extra traffic)
+        "histogram": 4 * n,          # one read of the codes (standalone sort)
         "binning": 8 * n,            # one read + one write of the keys, per pass
         "unique": 4 * n + 4 * u,     # sorted codes in, unique codes out
-        "radix_tree": 4 * u + 11 * nb,  # codes in; prefix_n, 2 leaf flags, left_child, parent out
-        "edge_count_scan": 14 * nb,  # prefix_n + parent + parent's prefix_n in; counts + offsets out
-        "octree_emit": 13 * nb + 64 * o,  # counts, offsets, codes, prefix_n in; nodes out
-        "octree_link": 23 * nb + 8 * o,   # tree arrays in; child slot + mask per link out
+        "radix_tree": 4 * u + 15 * nb,  # codes in; prefix_n, 2 leaf flags, left_child, parent, edge count out
+        "scan": 8 * nb,              # edge counts in, offsets out
+        "octree": 21 * nb + 64 * o,  # counts, offsets, codes, prefix_n, links in; nodes out
     }
+
+
+def timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def profile_stages(ctx, fn, reps):
+    import torch
+
+    ctx.profile_reset()
+    ctx.profile(True)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    return {k: (ms / cnt, cnt // reps) for k, (ms, cnt) in rep.items()}
 
 
 def run_cpu_baseline(pts, reps=2):
@@ -61,6 +95,133 @@ def run_cpu_baseline(pts, reps=2):
     }
 
 
+def bench_sort_u32(ctx, codes, steps):
+    import torch
+
+    n = codes.shape[0]
+    keys = codes.clone()
+    alt = torch.empty_like(codes)
+
+    def step():
+        keys.copy_(codes)
+        ctx.sort(keys, alt)
+
+    prof = profile_stages(ctx, step, steps)
+    ms = prof["histogram"][0] + prof["binning"][0] * prof["binning"][1]
+    return {
+        "workload": f"2^{n.bit_length() - 1} u32 Morton codes of the headline cloud, keys only, 4 x 8-bit passes",
+        "ms": round(ms, 4),
+        "histogram_ms": round(prof["histogram"][0], 4),
+        "pass_ms": round(prof["binning"][0], 4),
+        "alg_bytes": 36 * n,
+        "GBps": round(36 * n / (ms * 1e-3) / 1e9, 1),
+        "frac": round(36 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+
+def bench_sort_pairs_u64(ctx, log2n, steps):
+    import torch
+
+    import tc
+
+    n = 1 << log2n
+    pts = torch.from_numpy(tc.gen_uniform(n, 0xC0FFEE03, 2097152.0)).cuda()
+    keys0 = torch.empty(n, dtype=torch.int64, device="cuda")
+    ctx.morton63(pts, keys0, 0.0, 2097152.0)
+    del pts
+    vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
+    keys, vals = keys0.clone(), vals0.clone()
+    ka, va = torch.empty_like(keys), torch.empty_like(vals)
+
+    def step():
+        keys.copy_(keys0)
+        vals.copy_(vals0)
+        ctx.sort(keys, ka, vals, va)
+
+    prof = profile_stages(ctx, step, steps)
+    ms = prof["histogram"][0] + prof["binning"][0] * prof["binning"][1]
+    ok = bool(torch.all(keys[1:] >= keys[:-1]).item())
+    alg = 200 * n
+    return {
+        "workload": f"2^{log2n} 63-bit Morton keys (xyz uniform in [0,2^21)^3) + u32 index payload, 8 x 8-bit passes",
+        "ms": round(ms, 4),
+        "pass_ms": round(prof["binning"][0], 4),
+        "alg_bytes": alg,
+        "GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+        "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "sorted": ok,
+    }
+
+
+def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04):
+    import numpy as np
+    import torch
+
+    import tc
+
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 20), n_clouds))).astype(np.uint32)
+    total = int(sizes.sum())
+    host = np.empty((total, 4), np.float32)
+    off = 0
+    for i, s in enumerate(sizes):
+        s = int(s)
+        if i % 2 == 0:
+            host[off:off + s] = tc.gen_uniform(s, seed + i)
+        else:
+            host[off:off + s] = tc.gen_gaussian_mixture(s, seed + i, 16, 4.0)
+        off += s
+    pts = torch.from_numpy(host).cuda()
+    stream = torch.cuda.Stream()
+    batch = tc.Batch(ctx, n_clouds, total, 30)
+    with torch.cuda.stream(stream):
+        batch.set(sizes, stream=stream)
+        batch.capture(pts, stream=stream)
+    stream.synchronize()
+
+    # Batched: graph replays of the whole segmented pipeline.
+    lat = []
+    for i in range(steps + 2):
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            batch.replay(stream=stream)
+        stream.synchronize()
+        if i >= 2:
+            lat.append(time.perf_counter() - t0)
+    batched_s = float(np.mean(lat))
+
+    # Plain: one tc_build_tree per cloud, one cloud at a time.
+    tree = tc.Tree(ctx, int(sizes.max()), 30)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    views = [pts[offs[i]:offs[i + 1]] for i in range(n_clouds)]
+    plain_lat = []
+    for rep in range(max(1, steps // 4)):
+        t0 = time.perf_counter()
+        per_cloud = []
+        for i in range(n_clouds):
+            c0 = time.perf_counter()
+            tree.build(views[i], int(sizes[i]), 0.0, 1024.0, stream=stream)
+            stream.synchronize()
+            per_cloud.append(time.perf_counter() - c0)
+        plain_lat.append(time.perf_counter() - t0)
+    plain_s = float(np.mean(plain_lat))
+    tree.free()
+    batch.close()
+    return {
+        "workload": f"{n_clouds} clouds, sizes log-uniform in [2^10, 2^20] (total {total} points), alternating "
+                    "uniform / 16-cluster Gaussian (sigma 4) in [0,1024)^3, 30-bit codes",
+        "total_points": total,
+        "batched_ms": round(batched_s * 1e3, 3),
+        "batched_Mpoints_per_s": round(total / batched_s / 1e6, 1),
+        "batched_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "plain_loop_ms": round(plain_s * 1e3, 3),
+        "plain_Mpoints_per_s": round(total / plain_s / 1e6, 1),
+        "plain_cloud_p50_ms": round(float(np.percentile(per_cloud, 50)) * 1e3, 4),
+        "plain_cloud_p99_ms": round(float(np.percentile(per_cloud, 99)) * 1e3, 4),
+        "speedup": round(plain_s / batched_s, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,14 +230,14 @@ def main():
     ap.add_argument("--log2n", type=int, default=26)
     ap.add_argument("--seed", type=int, default=0xC0FFEE02)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline build only")
     ap.add_argument("--profile-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(clouds are independent, see the sharding design)", file=sys.stderr)
+              "(clouds are independent; the multi-GPU design shards whole clouds)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import tc
@@ -86,49 +247,30 @@ def main():
     ctx = tc.Context(0)
     pts = torch.from_numpy(pts_h).cuda()
     tree = tc.Tree(ctx, n, 30)
-    stream = torch.cuda.current_stream()
 
-    for _ in range(args.warmup):
-        tree.build(pts, n, 0.0, 1024.0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tree.build(pts, n, 0.0, 1024.0)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    step_s = timed(lambda: tree.build(pts, n, 0.0, 1024.0), args.steps, args.warmup)
     counts = tree.read_counts()
     u, o = counts["n_unique"], counts["n_nodes"]
-    ms_per_step = elapsed / args.steps * 1e3
+    ms_per_step = step_s * 1e3
 
-    # Per-kernel durations: HIP events around every launch, on the launch stream.
-    ctx.profile_reset()
-    ctx.profile(True)
-    for _ in range(args.profile_steps):
-        tree.build(pts, n, 0.0, 1024.0)
-    torch.cuda.synchronize()
-    rep = ctx.profile_report()
-    ctx.profile(False)
+    prof = profile_stages(ctx, lambda: tree.build(pts, n, 0.0, 1024.0), args.profile_steps)
     alg = algorithmic_bytes(n, u, o)
     stages = {}
-    for name, (tot_ms, cnt) in rep.items():
-        avg = tot_ms / cnt
-        key = "edge_count_scan" if name == "edge_count_scan" else name
-        entry = {"avg_ms": round(avg, 4), "launches_per_step": cnt // args.profile_steps}
-        if key in alg:
-            entry["alg_bytes"] = alg[key]
-            entry["GBps"] = round(alg[key] / (avg * 1e-3) / 1e9, 1)
+    for name, (avg, per_step) in prof.items():
+        entry = {"avg_ms": round(avg, 4), "launches_per_step": per_step}
+        if name in alg:
+            entry["alg_bytes"] = alg[name]
+            entry["GBps"] = round(alg[name] / (avg * 1e-3) / 1e9, 1)
             entry["roofline_frac"] = round(entry["GBps"] / HBM_PEAK_GBS, 4)
         stages[name] = entry
     dom = max((k for k in stages if "alg_bytes" in stages[k]),
               key=lambda k: stages[k]["avg_ms"] * stages[k]["launches_per_step"])
     d = stages[dom]
-    sort_ms = stages["histogram"]["avg_ms"] + 4 * stages["binning"]["avg_ms"]
-    build_bytes = sum(alg[k] * (4 if k == "binning" else 1) for k in alg)
-    value = n / (ms_per_step * 1e-3) / 1e6
+    build_bytes = sum(alg[k] * stages[k]["launches_per_step"] for k in alg if k in stages)
 
     out = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": round(n / step_s / 1e6, 2),
         "unit": "Mpoints/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -160,19 +302,24 @@ def main():
         },
         "build_roofline": {
             "alg_bytes": build_bytes,
-            "GBps": round(build_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-            "frac": round(build_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        },
-        "sort": {
-            "ms": round(sort_ms, 4),
-            "alg_bytes": 36 * n,
-            "frac": round(36 * n / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "GBps": round(build_bytes / step_s / 1e9, 1),
+            "frac": round(build_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
         },
         "stages": stages,
     }
+    tree.free()
+    if not args.no_extras:
+        codes = torch.empty(n, dtype=torch.int32, device="cuda")
+        ctx.morton30(pts, codes, 0.0, 1024.0)
+        out["sort"] = bench_sort_u32(ctx, codes, args.profile_steps)
+        del codes
+        del pts
+        torch.cuda.empty_cache()
+        out["sort_pairs_u64"] = bench_sort_pairs_u64(ctx, 27, 2)
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batched(ctx, 1024, max(4, args.steps // 2))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(pts_h)
-    tree.free()
     ctx.close()
     print(json.dumps(out), flush=True)
 

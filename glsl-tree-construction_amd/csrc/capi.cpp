@@ -325,8 +325,7 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     if (t->code_bits == 30) {
         uint32_t* sorted = static_cast<uint32_t*>(t->sorted);
         uint32_t* uniq = static_cast<uint32_t*>(t->unique);
-        TC_TRY(tc_morton30(ctx, pts, sorted, n, min_coord, range, stream));
-        TC_TRY(tc_sort_keys_u32(ctx, sorted, static_cast<uint32_t*>(t->alt), n, stream));
+        TC_TRY(tc_morton_sort_u32(ctx, pts, sorted, static_cast<uint32_t*>(t->alt), n, min_coord, range, stream));
         TC_TRY(tc_unique_u32(ctx, sorted, uniq, n, t->d_counts, stream));
         TC_TRY(tc_build_radix_tree_ec_u32(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
                                           t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
@@ -338,8 +337,7 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     } else {
         uint64_t* sorted = static_cast<uint64_t*>(t->sorted);
         uint64_t* uniq = static_cast<uint64_t*>(t->unique);
-        TC_TRY(tc_morton63(ctx, pts, sorted, n, min_coord, range, stream));
-        TC_TRY(tc_sort_keys_u64(ctx, sorted, static_cast<uint64_t*>(t->alt), n, stream));
+        TC_TRY(tc_morton_sort_u64(ctx, pts, sorted, static_cast<uint64_t*>(t->alt), n, min_coord, range, stream));
         TC_TRY(tc_unique_u64(ctx, sorted, uniq, n, t->d_counts, stream));
         TC_TRY(tc_build_radix_tree_ec_u64(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
                                           t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));

@@ -12,22 +12,24 @@
 //    the last block to finish turns the counts into exclusive prefixes and
 //    clears the accumulator for the next call.  It also clears the pass-0
 //    look-back words and the per-pass tile counters.
-//  * binning pass: a tile of BLOCK*KPT keys per workgroup, tile ids handed out
-//    by an atomic counter (forward progress for the look-back).  Each wave
-//    ranks its 64*KPT keys with 64-lane ballots (8 ballots give the peer mask
-//    of lanes holding the same digit), so the rank is stable.  Per-wave digit
-//    counters live in LDS; a block scan turns them into tile-local offsets; keys
-//    are staged in LDS in digit order and written out in runs (coalesced
-//    stores).  Look-back words are u32 (2-bit flag, 30-bit count) published and
-//    read with agent-scope atomics, since the 8 XCDs' L2 caches are not
-//    coherent with each other; one thread per digit walks back.  Each pass
-//    clears the other look-back region for the next pass.
+//    In the pipeline the histogram is fused into the Morton kernel instead.
+//  * binning pass: a tile of BLOCK*KPT keys per workgroup (tile = workgroup
+//    id).  Each wave ranks its 64*KPT keys with 64-lane ballots (8 ballots give
+//    the mask of lanes holding the same digit), so the rank is stable.  Per-wave
+//    digit counters live in LDS; a block scan turns them into tile-local
+//    offsets; keys are staged in LDS in digit order and written out in runs
+//    (coalesced stores).  Look-back words are u32 (2-bit flag, 30-bit count)
+//    published and read with agent-scope atomics, since the 8 XCDs' L2 caches
+//    are not coherent with each other; one thread per digit walks back, four
+//    predecessors per round trip, with a bounded wait and a self-computed
+//    fall-back.  Each pass clears the other look-back region for the next pass.
 #include "sort_core.hpp"
 
 namespace {
 
 constexpr int HIST_BLOCK = 256;
-constexpr int HIST_REP = 2;  // LDS sub-histogram copies per block (waves alternate)
+constexpr int HIST_REP = 4;  // LDS sub-histogram copies per block (one per wave)
+constexpr int HG_ITEMS = 4;  // 16-byte key loads in flight per thread in the histogram kernel
 
 template <typename K> struct KeyVec;
 template <> struct KeyVec<uint32_t> { using V = uint4; static constexpr int N = 4; };
@@ -42,6 +44,35 @@ TC_DEV void vec_get<uint32_t>(const uint4& v, uint32_t* out) {
 template <>
 TC_DEV void vec_get<uint64_t>(const ulonglong2& v, uint64_t* out) {
     out[0] = v.x; out[1] = v.y;
+}
+
+// Merge a block's LDS sub-histograms into the device accumulator; the last
+// block to finish turns the counts into exclusive prefixes (digit-major) and
+// clears the accumulator and the ticket for the next call.
+template <int DIG>
+TC_DEV void hist_finish(const uint32_t* s_h, uint32_t* s_tmp, uint32_t* s_last, uint32_t* __restrict__ acc,
+                        uint32_t* __restrict__ done, uint32_t* __restrict__ hist_out) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b = tid; b < DIG * 256; b += HIST_BLOCK) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int r = 0; r < HIST_REP; ++r) s += s_h[r * DIG * 256 + b];
+        if (s) atomicAdd(&acc[b], s);
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) *s_last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1u : 0u;
+    __syncthreads();
+    if (!*s_last) return;
+    __threadfence();
+    for (int d = 0; d < DIG; ++d) {
+        const uint32_t c = ld_acquire_agent(&acc[d * 256 + tid]);
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<HIST_BLOCK>(c, s_tmp, &tot);
+        hist_out[d * 256 + tid] = ex;
+        st_agent(&acc[d * 256 + tid], 0u);
+    }
+    if (tid == 0) st_agent(done, 0u);
 }
 
 template <typename K>
@@ -63,7 +94,24 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
     const uint32_t gstride = gridDim.x * HIST_BLOCK;
     const uint32_t nvec = n / VN;
     const typename KeyVec<K>::V* kv = reinterpret_cast<const typename KeyVec<K>::V*>(keys);
-    for (uint32_t v = gtid; v < nvec; v += gstride) {
+    // HG_ITEMS 16-byte loads in flight per thread, then the vectors' keys are counted.
+    constexpr uint32_t CHUNK = HIST_BLOCK * HG_ITEMS;
+    const uint32_t full = nvec / CHUNK;
+    for (uint32_t c = blockIdx.x; c < full; c += gridDim.x) {
+        typename KeyVec<K>::V v[HG_ITEMS];
+#pragma unroll
+        for (int u = 0; u < HG_ITEMS; ++u) v[u] = kv[c * CHUNK + u * HIST_BLOCK + tid];
+#pragma unroll
+        for (int u = 0; u < HG_ITEMS; ++u) {
+            K k[VN];
+            vec_get<K>(v[u], k);
+#pragma unroll
+            for (int j = 0; j < VN; ++j)
+#pragma unroll
+                for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((k[j] >> (8 * d)) & 255u)], 1u);
+        }
+    }
+    for (uint32_t v = full * CHUNK + gtid; v < nvec; v += gstride) {
         K k[VN];
         vec_get<K>(kv[v], k);
 #pragma unroll
@@ -79,29 +127,69 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
     // Clear the pass-0 look-back words and the tile counters for the passes.
     for (uint32_t w = gtid; w < zero_words; w += gstride) zero_ptr[w] = 0u;
     __syncthreads();
+    hist_finish<DIG>(&s_h[0][0], s_tmp, &s_last, acc, done, hist_out);
+}
 
-    for (uint32_t b = tid; b < DIG * 256; b += HIST_BLOCK) {
-        uint32_t s = 0;
+template <typename K>
+TC_DEV K encode_point(const float4& p, float min_coord, float range) {
+    if constexpr (sizeof(K) == 4) {
+        return d_split3_10(d_quantize(p.x, min_coord, range, 1024.0f)) |
+               (d_split3_10(d_quantize(p.y, min_coord, range, 1024.0f)) << 1) |
+               (d_split3_10(d_quantize(p.z, min_coord, range, 1024.0f)) << 2);
+    } else {
+        return d_split3_21(d_quantize(p.x, min_coord, range, 2097152.0f)) |
+               (d_split3_21(d_quantize(p.y, min_coord, range, 2097152.0f)) << 1) |
+               (d_split3_21(d_quantize(p.z, min_coord, range, 2097152.0f)) << 2);
+    }
+}
+
+// Morton codes with the sort's global histogram fused in: the codes are counted
+// while they are in registers, so the sort does not read them an extra time
+// (k_ComputeMortonCode, shader/morton.comp:60-67 + k_GlobalHistogram,
+// shader/histogram.comp:103-195).  The grid is small (the histogram merge costs
+// one device atomic per bin per block), so each thread keeps MH_ITEMS point
+// loads in flight to cover HBM latency.
+constexpr int MH_ITEMS = 8;
+
+template <typename K>
+__global__ __launch_bounds__(HIST_BLOCK) void k_morton_hist(
+    const float4* __restrict__ pts, K* __restrict__ codes, uint32_t n, float min_coord, float range,
+    uint32_t* __restrict__ acc, uint32_t* __restrict__ done, uint32_t* __restrict__ hist_out,
+    uint32_t* __restrict__ zero_ptr, uint32_t zero_words) {
+    constexpr int DIG = (int)sizeof(K);
+    constexpr uint32_t CHUNK = HIST_BLOCK * MH_ITEMS;
+    __shared__ uint32_t s_h[HIST_REP][DIG * 256];
+    __shared__ uint32_t s_tmp[HIST_BLOCK / 64 + 2];
+    __shared__ uint32_t s_last;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < HIST_REP * DIG * 256; i += HIST_BLOCK) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+    uint32_t* h = s_h[(tid / 64) % HIST_REP];
+    const uint32_t full = n / CHUNK;
+    for (uint32_t c = blockIdx.x; c < full; c += gridDim.x) {
+        const uint32_t b0 = c * CHUNK + tid;
+        float4 p[MH_ITEMS];
 #pragma unroll
-        for (int r = 0; r < HIST_REP; ++r) s += s_h[r][b];
-        if (s) atomicAdd(&acc[b], s);
+        for (int u = 0; u < MH_ITEMS; ++u) p[u] = pts[b0 + u * HIST_BLOCK];
+#pragma unroll
+        for (int u = 0; u < MH_ITEMS; ++u) {
+            const K code = encode_point<K>(p[u], min_coord, range);
+            codes[b0 + u * HIST_BLOCK] = code;
+#pragma unroll
+            for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((code >> (8 * d)) & 255u)], 1u);
+        }
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1u : 0u;
-    __syncthreads();
-    if (!s_last) return;
-
-    // Last block: counts -> exclusive prefixes (digit-major), then clear.
-    __threadfence();
-    for (int d = 0; d < DIG; ++d) {
-        const uint32_t c = ld_acquire_agent(&acc[d * 256 + tid]);
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<HIST_BLOCK>(c, s_tmp, &tot);
-        hist_out[d * 256 + tid] = ex;
-        st_agent(&acc[d * 256 + tid], 0u);
+    const uint32_t gtid = blockIdx.x * HIST_BLOCK + tid;
+    const uint32_t gstride = gridDim.x * HIST_BLOCK;
+    for (uint32_t i = full * CHUNK + gtid; i < n; i += gstride) {
+        const K code = encode_point<K>(pts[i], min_coord, range);
+        codes[i] = code;
+#pragma unroll
+        for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((code >> (8 * d)) & 255u)], 1u);
     }
-    if (tid == 0) st_agent(done, 0u);
+    for (uint32_t w = gtid; w < zero_words; w += gstride) zero_ptr[w] = 0u;
+    __syncthreads();
+    hist_finish<DIG>(&s_h[0][0], s_tmp, &s_last, acc, done, hist_out);
 }
 
 template <typename K, bool PAIRS, int BLOCK, int KPT>
@@ -196,8 +284,11 @@ int histogram_impl(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist_out, u
     return TC_OK;
 }
 
+// Sort keys (and values) in place.  With pts != NULL the keys are first
+// computed from the points by the fused Morton + histogram kernel.
 template <typename K>
-int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint32_t n, hipStream_t s) {
+int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint32_t n, hipStream_t s,
+              const float* pts = nullptr, float min_coord = 0.f, float range = 1.f) {
     if (!ctx || (n && (!keys || !alt))) return TC_ERR_INVALID;
     if (vals && !valt) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
@@ -208,8 +299,17 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
     uint32_t tiles;
     TC_TRY(sort_scratch(ctx, n, DIG, &ss, &tiles));
     const uint32_t region_words = tiles * 256;
-    // Histogram also clears the tile counters and look-back region 0.
-    TC_TRY(histogram_impl<K>(ctx, keys, n, ss.hist, ss.ctr, 64 + region_words, s));
+    // The histogram kernel also clears the tile counters and look-back region 0.
+    if (pts) {
+        ProfScope prof(ctx, s, K_MORTON);
+        hipLaunchKernelGGL((k_morton_hist<K>), dim3(hist_grid(ctx, n)), dim3(HIST_BLOCK), 0, s,
+                           reinterpret_cast<const float4*>(pts), keys, n, min_coord, range,
+                           ctx->small + SMALL_HIST_ACC, ctx->small + SMALL_HIST_DONE, ss.hist, ss.ctr,
+                           64 + region_words);
+        TC_LAUNCH_CHECK();
+    } else {
+        TC_TRY(histogram_impl<K>(ctx, keys, n, ss.hist, ss.ctr, 64 + region_words, s));
+    }
     K* src = keys;
     K* dst = alt;
     uint32_t* vsrc = vals;
@@ -289,6 +389,16 @@ extern "C" int tc_sort_pairs_u32(tc_ctx* ctx, uint32_t* keys, uint32_t* vals, ui
                                  uint32_t n, tc_stream stream) {
     if (!vals) return TC_ERR_INVALID;
     return sort_impl<uint32_t>(ctx, keys, vals, alt, valt, n, as_stream(stream));
+}
+extern "C" int tc_morton_sort_u32(tc_ctx* ctx, const float* pts, uint32_t* keys, uint32_t* alt, uint32_t n,
+                                  float min_coord, float range, tc_stream stream) {
+    if (n && (!pts || (reinterpret_cast<uintptr_t>(pts) & 15u))) return TC_ERR_INVALID;
+    return sort_impl<uint32_t>(ctx, keys, nullptr, alt, nullptr, n, as_stream(stream), pts, min_coord, range);
+}
+extern "C" int tc_morton_sort_u64(tc_ctx* ctx, const float* pts, uint64_t* keys, uint64_t* alt, uint32_t n,
+                                  float min_coord, float range, tc_stream stream) {
+    if (n && (!pts || (reinterpret_cast<uintptr_t>(pts) & 15u))) return TC_ERR_INVALID;
+    return sort_impl<uint64_t>(ctx, keys, nullptr, alt, nullptr, n, as_stream(stream), pts, min_coord, range);
 }
 extern "C" int tc_sort_keys_u64(tc_ctx* ctx, uint64_t* keys, uint64_t* alt, uint32_t n, tc_stream stream) {
     return sort_impl<uint64_t>(ctx, keys, nullptr, alt, nullptr, n, as_stream(stream));

@@ -20,10 +20,14 @@ TC_DEV uint64_t match_digit8(uint32_t d, uint64_t live) {
     uint32_t diff_lo = 0u, diff_hi = 0u;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-        const uint32_t rep = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        uint32_t rep = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);
+        // Opaque to the optimiser, which would otherwise test the bit again
+        // with a shift + compare instead of comparing rep (no instruction).
+        asm("" : "+v"(rep));
         const uint64_t m = __ballot(rep != 0u);
-        diff_lo |= rep ^ (uint32_t)m;
-        diff_hi |= rep ^ (uint32_t)(m >> 32);
+        // (rep ^ m) | diff in one v_bitop3 per half (truth table 0xde, operands a=rep, b=diff, c=m).
+        diff_lo = __builtin_amdgcn_bitop3_b32(rep, diff_lo, (uint32_t)m, 0xde);
+        diff_hi = __builtin_amdgcn_bitop3_b32(rep, diff_hi, (uint32_t)(m >> 32), 0xde);
     }
     return live & ~(((uint64_t)diff_hi << 32) | diff_lo);
 }
@@ -38,7 +42,6 @@ struct OnesweepShared {
     static constexpr int WAVES = BLOCK / 64;
     static constexpr uint32_t TILE = BLOCK * KPT;
     uint32_t whist[WAVES * 256];
-    uint32_t start[256];
     uint32_t gbase[256];
     uint32_t tmp[WAVES + 2];
     uint32_t fb[256];  // fall-back digit counts
@@ -57,12 +60,12 @@ struct OnesweepShared {
 // look-back row of this tile, cleared here.
 // seg_base is the index of the segment's first key (the fall-back counts the
 // keys in [seg_base, base)); spin_limit bounds the look-back wait.
-template <typename K, bool PAIRS, int BLOCK, int KPT>
-TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict__ sk, K* __restrict__ dk,
-                          const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
-                          uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first, uint32_t seg_base,
-                          const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
-                          uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
+template <typename K, bool PAIRS, int BLOCK, int KPT, bool FULL>
+TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict__ sk, K* __restrict__ dk,
+                               const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
+                               uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first, uint32_t seg_base,
+                               const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
+                               uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
     constexpr int WAVES = BLOCK / 64;
     constexpr uint32_t TILE = BLOCK * KPT;
     constexpr uint32_t WAVE_TILE = 64 * KPT;
@@ -73,7 +76,7 @@ TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    const bool full = cnt == TILE;
+    constexpr bool full = FULL;  // full tiles (cnt == TILE) skip every bounds test
     const uint32_t wofs = wave * WAVE_TILE + lane;  // tile-relative index of key 0 of this lane
 
     for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) sm.whist[i] = 0u;
@@ -81,7 +84,7 @@ TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict
     K keys[KPT];
     uint32_t vals[KPT];
     uint32_t pos[KPT];
-    if (full) {
+    if constexpr (FULL) {
 #pragma unroll
         for (int i = 0; i < KPT; ++i) keys[i] = sk[base + wofs + i * 64];
         if (PAIRS) {
@@ -135,7 +138,11 @@ TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict
     }
     uint32_t tile_total;
     const uint32_t start = block_exclusive_scan<BLOCK>(total_d, sm.tmp, &tile_total);
-    if (tid < 256) sm.start[tid] = start;
+    if (tid < 256) {
+        // whist[w][d] becomes the tile-local position of wave w's first digit-d key.
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) sm.whist[w * 256 + tid] += start;
+    }
     __syncthreads();
 
     // Stage keys in LDS in (digit, original order) order.
@@ -144,7 +151,7 @@ TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict
         const bool valid = full || (wofs + i * 64 < cnt);
         if (valid) {
             const uint32_t d = digit_of<K>(keys[i], shift);
-            const uint32_t lp = sm.start[d] + sm.whist[wave * 256 + d] + pos[i];
+            const uint32_t lp = sm.whist[wave * 256 + d] + pos[i];
             s_keys[lp] = keys[i];
             pos[i] = lp;
         }
@@ -207,7 +214,7 @@ TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
         const uint32_t j = tid + i * BLOCK;
-        if (j < cnt) {
+        if (full || j < cnt) {
             const K key = s_keys[j];
             dst[i] = sm.gbase[digit_of<K>(key, shift)] + j;
             dk[dst[i]] = key;
@@ -224,7 +231,22 @@ TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict
 #pragma unroll
         for (int i = 0; i < KPT; ++i) {
             const uint32_t j = tid + i * BLOCK;
-            if (j < cnt) dv[dst[i]] = s_vals[j];
+            if (full || j < cnt) dv[dst[i]] = s_vals[j];
         }
     }
+}
+
+// Full tiles take the variant without bounds tests (cnt is uniform per workgroup).
+template <typename K, bool PAIRS, int BLOCK, int KPT>
+TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict__ sk, K* __restrict__ dk,
+                          const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
+                          uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first, uint32_t seg_base,
+                          const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
+                          uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
+    if (cnt == OnesweepShared<K, BLOCK, KPT>::TILE)
+        onesweep_tile_impl<K, PAIRS, BLOCK, KPT, true>(sm, sk, dk, sv, dv, base, cnt, shift, tile, first, seg_base,
+                                                       prefix, add, lb_cur, lb_next, spin_limit);
+    else
+        onesweep_tile_impl<K, PAIRS, BLOCK, KPT, false>(sm, sk, dk, sv, dv, base, cnt, shift, tile, first,
+                                                        seg_base, prefix, add, lb_cur, lb_next, spin_limit);
 }

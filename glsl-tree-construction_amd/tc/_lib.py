@@ -126,6 +126,8 @@ SIGNATURES = {
     "tc_sort_keys_u32": (I, [P, P, P, U32, S]),
     "tc_sort_pairs_u32": (I, [P, P, P, P, P, U32, S]),
     "tc_sort_keys_u64": (I, [P, P, P, U32, S]),
+    "tc_morton_sort_u32": (I, [P, P, P, P, U32, F, F, S]),
+    "tc_morton_sort_u64": (I, [P, P, P, P, U32, F, F, S]),
     "tc_sort_pairs_u64": (I, [P, P, P, P, P, U32, S]),
     "tc_unique_u32": (I, [P, P, P, U32, P, S]),
     "tc_unique_u64": (I, [P, P, P, U32, P, S]),

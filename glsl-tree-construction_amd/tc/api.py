@@ -108,6 +108,14 @@ class Context:
         check(lib().tc_morton63(self._h, _t_ptr(pts), _t_ptr(codes), pts.shape[0], min_coord, range_,
                                 _stream(stream)), "tc_morton63")
 
+    def morton_sort(self, pts, keys, alt, min_coord, range_, n=None, stream=None):
+        """Morton codes of ``pts`` (30-bit into int32 ``keys``, 63-bit into int64) sorted ascending;
+        the histogram of the first sort pass is fused into the encoding kernel."""
+        n = pts.shape[0] if n is None else n
+        fn = lib().tc_morton_sort_u64 if keys.element_size() == 8 else lib().tc_morton_sort_u32
+        check(fn(self._h, _t_ptr(pts), _t_ptr(keys), _t_ptr(alt), n, min_coord, range_, _stream(stream)),
+              "tc_morton_sort")
+
     def morton30_decode(self, codes, corners, min_coord, range_, stream=None):
         check(lib().tc_morton30_decode(self._h, _t_ptr(codes), _t_ptr(corners), codes.shape[0], min_coord,
                                        range_, _stream(stream)), "tc_morton30_decode")

@@ -140,6 +140,14 @@ int tc_sort_keys_u32(tc_ctx* ctx, uint32_t* keys, uint32_t* keys_alt, uint32_t n
 int tc_sort_pairs_u32(tc_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                       uint32_t* vals_alt, uint32_t n, tc_stream stream);
 int tc_sort_keys_u64(tc_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, uint32_t n, tc_stream stream);
+/* Morton encode + sort in one call: the codes of pts_xyzw land in keys, sorted;
+ * the sort's histogram is accumulated by the Morton kernel itself, so the
+ * codes are not re-read for it (same result as tc_morton30/63 followed by
+ * tc_sort_keys_u32/64). */
+int tc_morton_sort_u32(tc_ctx* ctx, const float* pts_xyzw, uint32_t* keys, uint32_t* keys_alt, uint32_t n,
+                       float min_coord, float range, tc_stream stream);
+int tc_morton_sort_u64(tc_ctx* ctx, const float* pts_xyzw, uint64_t* keys, uint64_t* keys_alt, uint32_t n,
+                       float min_coord, float range, tc_stream stream);
 int tc_sort_pairs_u64(tc_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                       uint32_t* vals_alt, uint32_t n, tc_stream stream);
 

@@ -155,6 +155,21 @@ def test_sort_launch_shapes(gpu_ctx, oracle, block, kpt):
         gpu_ctx.set_sort_params(*prev)
 
 
+@pytest.mark.parametrize("n", [1, 2, 1000, 300_001, 1 << 20])
+def test_morton_sort_fused_histogram(gpu_ctx, oracle, tclib, n):
+    """tc_morton_sort_* (histogram of pass 0 fused into the encoder) == encode, then sort."""
+    pts = tclib.gen_gaussian_mixture(n, 700 + n, 16, 4.0)
+    pts[::11, :3] = -5.0  # clamped codes, many ties
+    want30 = oracle.radix_sort(oracle.compute_morton(pts, 0.0, 1024.0))
+    k = empty(n, np.uint32)
+    gpu_ctx.morton_sort(dev(pts), k, empty(n, np.uint32), 0.0, 1024.0)
+    assert np.array_equal(host(k, np.uint32), want30)
+    want63 = oracle.radix_sort(oracle.compute_morton63(pts, 0.0, 1024.0))
+    k = empty(n, np.uint64)
+    gpu_ctx.morton_sort(dev(pts), k, empty(n, np.uint64), 0.0, 1024.0)
+    assert np.array_equal(host(k, np.uint64), want63)
+
+
 def test_binning_pass_api(gpu_ctx, oracle):
     r = np.random.default_rng(9)
     keys = r.integers(0, 1 << 32, size=100_000, dtype=np.uint64).astype(np.uint32)
