@@ -147,8 +147,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
                        (d_split3_21(d_quantize(q.z, mn, rg, 2097152.0f)) << 2);
             }
             codes[i] = code;
-#pragma unroll
-            for (int d = 0; d < DIG; ++d) atomicAdd(&s_h[d * 256 + (uint32_t)((code >> (8 * d)) & 255u)], 1u);
+            lds_count_digits<K, DIG>(s_h, code);
         }
         __syncthreads();
     }

@@ -107,8 +107,7 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
             vec_get<K>(v[u], k);
 #pragma unroll
             for (int j = 0; j < VN; ++j)
-#pragma unroll
-                for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((k[j] >> (8 * d)) & 255u)], 1u);
+                lds_count_digits<K, DIG>(h, k[j]);
         }
     }
     for (uint32_t v = full * CHUNK + gtid; v < nvec; v += gstride) {
@@ -116,13 +115,11 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
         vec_get<K>(kv[v], k);
 #pragma unroll
         for (int j = 0; j < VN; ++j)
-#pragma unroll
-            for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((k[j] >> (8 * d)) & 255u)], 1u);
+            lds_count_digits<K, DIG>(h, k[j]);
     }
     for (uint32_t i = nvec * VN + gtid; i < n; i += gstride) {
         const K k = keys[i];
-#pragma unroll
-        for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((k >> (8 * d)) & 255u)], 1u);
+        lds_count_digits<K, DIG>(h, k);
     }
     // Clear the pass-0 look-back words and the tile counters for the passes.
     for (uint32_t w = gtid; w < zero_words; w += gstride) zero_ptr[w] = 0u;
@@ -175,8 +172,7 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_morton_hist(
         for (int u = 0; u < MH_ITEMS; ++u) {
             const K code = encode_point<K>(p[u], min_coord, range);
             codes[b0 + u * HIST_BLOCK] = code;
-#pragma unroll
-            for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((code >> (8 * d)) & 255u)], 1u);
+            lds_count_digits<K, DIG>(h, code);
         }
     }
     const uint32_t gtid = blockIdx.x * HIST_BLOCK + tid;
@@ -184,8 +180,7 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_morton_hist(
     for (uint32_t i = full * CHUNK + gtid; i < n; i += gstride) {
         const K code = encode_point<K>(pts[i], min_coord, range);
         codes[i] = code;
-#pragma unroll
-        for (int d = 0; d < DIG; ++d) atomicAdd(&h[d * 256 + (uint32_t)((code >> (8 * d)) & 255u)], 1u);
+        lds_count_digits<K, DIG>(h, code);
     }
     for (uint32_t w = gtid; w < zero_words; w += gstride) zero_ptr[w] = 0u;
     __syncthreads();

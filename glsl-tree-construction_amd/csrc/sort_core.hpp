@@ -67,7 +67,6 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
                                const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
                                uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
     constexpr int WAVES = BLOCK / 64;
-    constexpr uint32_t TILE = BLOCK * KPT;
     constexpr uint32_t WAVE_TILE = 64 * KPT;
     static_assert(BLOCK >= 256, "one thread per digit");
     K* s_keys = reinterpret_cast<K*>(sm.stage);

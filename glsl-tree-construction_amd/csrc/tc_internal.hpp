@@ -162,6 +162,25 @@ TC_DEV uint32_t mbcnt64(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Adds the DIG 8-bit digits of `key` to the LDS histograms h[d * 256 + digit].
+// When every active lane of the wave holds the same digit (the high digits of
+// sorted or spatially coherent input), one lane adds the wave's count: 64
+// same-address LDS atomics would otherwise serialise.
+template <typename K, int DIG>
+TC_DEV void lds_count_digits(uint32_t* h, K key) {
+    const uint64_t active = __ballot(true);
+#pragma unroll
+    for (int d = 0; d < DIG; ++d) {
+        const uint32_t v = (uint32_t)((key >> (8 * d)) & 255u);
+        const uint32_t v0 = __builtin_amdgcn_readfirstlane(v);
+        if (__ballot(v == v0) == active) {
+            if (mbcnt64(active) == 0u) atomicAdd(&h[d * 256 + v0], (uint32_t)__popcll(active));
+        } else {
+            atomicAdd(&h[d * 256 + v], 1u);
+        }
+    }
+}
+
 TC_DEV uint32_t ld_acquire_agent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
