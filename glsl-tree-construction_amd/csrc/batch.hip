@@ -29,7 +29,8 @@ struct tc_batch {
     int code_bits = 30;
     int block = 256, kpt = 16;
     uint32_t tile = 4096;
-    // Device plan block (one upload per batch): [desc 8][pt_off C+1][first_tile C+1][min C][range C].
+    // Device plan block (one upload per batch): [desc 8][pt_off C+1][first_tile C+1][min C][range C]
+    // [in_ptr C x u64][out C x tc_tree_ptrs].  desc: [0] clouds [1] tiles [2] points [3] flags.
     uint32_t* plan = nullptr;
     size_t plan_words = 0;
     uint32_t* h_plan = nullptr;  // pinned staging
@@ -56,6 +57,7 @@ struct tc_batch {
     uint64_t node_cap = 0;
     // Host copy of the current plan shape.
     uint32_t n_clouds = 0, n_points = 0, n_tiles = 0;
+    bool in_ptrs = false;  // the current plan reads clouds from per-cloud buffers
     // Graph.
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -68,24 +70,36 @@ namespace {
 constexpr int SEG_BLOCK = 256;
 
 // Plan views.
+constexpr uint32_t PLAN_IN_PTRS = 1u;   // desc[3]: cloud c is read from in_ptr[c]
+constexpr uint32_t PLAN_OUT_PTRS = 2u;  // desc[3]: outputs are copied to out[c]
+
 struct Plan {
-    const uint32_t* desc;  // [0] n_clouds [1] n_tiles [2] n_points
+    const uint32_t* desc;  // [0] n_clouds [1] n_tiles [2] n_points [3] flags
     const uint32_t* pt_off;
     const uint32_t* first_tile;
     const float* min_c;
     const float* range_c;
+    const uint64_t* in_ptr;
+    const tc_tree_ptrs* out;
 };
 
-Plan plan_view(const tc_batch* b) {
+size_t plan_words(uint32_t C) { return 8 + 2 * ((size_t)C + 1) + 2 * (size_t)C + 2 * (size_t)C + 18 * (size_t)C; }
+
+template <typename T>
+Plan plan_at(T* base, uint32_t C) {
     Plan p;
-    const uint32_t C = b->cap_clouds;
-    p.desc = b->plan;
-    p.pt_off = b->plan + 8;
+    p.desc = base;
+    p.pt_off = base + 8;
     p.first_tile = p.pt_off + (C + 1);
     p.min_c = reinterpret_cast<const float*>(p.first_tile + (C + 1));
     p.range_c = p.min_c + C;
+    // 8 + 2(C+1) + 2C words: an even count, so the 64-bit arrays are aligned.
+    p.in_ptr = reinterpret_cast<const uint64_t*>(p.range_c + C);
+    p.out = reinterpret_cast<const tc_tree_ptrs*>(p.in_ptr + C);
     return p;
 }
+
+Plan plan_view(const tc_batch* b) { return plan_at(b->plan, b->cap_clouds); }
 
 // Largest c in [0, n-1] with off[c] <= x (the non-empty segment holding x).
 TC_DEV uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t x) {
@@ -135,8 +149,10 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
         const uint32_t end_c = p.pt_off[c + 1];
         const uint32_t end = start + tile_size < end_c ? start + tile_size : end_c;
         const float mn = p.min_c[c], rg = p.range_c[c];
+        const uint32_t c0 = p.pt_off[c];
+        const float4* src = (p.desc[3] & PLAN_IN_PTRS) ? reinterpret_cast<const float4*>(p.in_ptr[c]) : pts + c0;
         for (uint32_t i = start + tid; i < end; i += SEG_BLOCK) {
-            const float4 q = pts[i];
+            const float4 q = src[i - c0];
             K code;
             if constexpr (sizeof(K) == 4) {
                 code = d_split3_10(d_quantize(q.x, mn, rg, 1024.0f)) | (d_split3_10(d_quantize(q.y, mn, rg, 1024.0f)) << 1) |
@@ -267,6 +283,43 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
     }
 }
 
+// Copy every cloud's outputs into its own buffers (tc_batch_set_io), in the
+// layout of a plain build: offsets rebased to the cloud's first node.  Exits at
+// once when the plan has no output buffers, so the launch can sit in a
+// captured graph unconditionally.
+template <typename K>
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_copyout(
+    Plan p, const uint32_t* __restrict__ ubase, const uint32_t* __restrict__ node_base, const K* __restrict__ uniq,
+    const uint8_t* __restrict__ prefix_n, const uint8_t* __restrict__ hll, const uint8_t* __restrict__ hlr,
+    const int32_t* __restrict__ left, const int32_t* __restrict__ parent, const uint32_t* __restrict__ ec,
+    const uint32_t* __restrict__ offsets, const tc_oct_node* __restrict__ nodes) {
+    if (!(p.desc[3] & PLAN_OUT_PTRS)) return;
+    const uint32_t C = p.desc[0];
+    const uint32_t stride = gridDim.x * SEG_BLOCK;
+    const uint32_t total_u = ubase[C];
+    for (uint32_t g = blockIdx.x * SEG_BLOCK + threadIdx.x; g < total_u; g += stride) {
+        const uint32_t c = seg_find(ubase, C, g);
+        const uint32_t b = ubase[c], nk = ubase[c + 1] - b, i = g - b;
+        const tc_tree_ptrs o = p.out[c];
+        static_cast<K*>(o.unique)[i] = uniq[g];
+        o.offsets[i] = offsets[g] - node_base[c];
+        if (i + 1 < nk) {
+            o.prefix_n[i] = prefix_n[g];
+            o.has_leaf_left[i] = hll[g];
+            o.has_leaf_right[i] = hlr[g];
+            o.left_child[i] = left[g];
+            o.parent[i] = parent[g];
+            o.edge_count[i] = ec[g];
+        }
+    }
+    const uint64_t total_q = 4ull * node_base[C];  // 16-byte quarters of the nodes
+    const uint4* src = reinterpret_cast<const uint4*>(nodes);
+    for (uint64_t g = (uint64_t)blockIdx.x * SEG_BLOCK + threadIdx.x; g < total_q; g += stride) {
+        const uint32_t c = seg_find(node_base, C, (uint32_t)(g >> 2));
+        reinterpret_cast<uint4*>(p.out[c].nodes)[g - 4ull * node_base[c]] = src[g];
+    }
+}
+
 template <typename K, int BLOCK, int KPT>
 void launch_seg_pass(tc_batch* b, hipStream_t s, uint32_t grid, const K* src, K* dst, uint32_t pass) {
     const Plan p = plan_view(b);
@@ -345,6 +398,9 @@ int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s
     hipLaunchKernelGGL(k_seg_octree<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, b->node_base,
                        b->nodes, ncap, b->offsets, b->ec, uniq, b->prefix_n, b->left, b->hll, b->hlr, link_leaves, p);
     TC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_seg_copyout<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p, b->ubase, b->node_base, uniq,
+                       b->prefix_n, b->hll, b->hlr, b->left, b->parent, b->ec, b->offsets, b->nodes);
+    TC_LAUNCH_CHECK();
     return TC_OK;
 }
 
@@ -386,7 +442,7 @@ extern "C" int tc_batch_create(tc_ctx* ctx, uint32_t cap_clouds, uint32_t cap_po
     const size_t C = cap_clouds, N = cap_points ? cap_points : 1;
     const size_t ks = code_bits == 30 ? 4 : 8;
     const int dig = code_bits == 30 ? 4 : 8;
-    b->plan_words = 8 + 2 * (C + 1) + 2 * C;
+    b->plan_words = plan_words(cap_clouds);
     // Node bound: per cloud it is tc_octree_node_bound(n_c); the sum over clouds
     // is at most the bound for the total plus one root per cloud.
     b->node_cap = tc_octree_node_bound(N, code_bits) + C;
@@ -426,18 +482,34 @@ extern "C" int tc_batch_destroy(tc_batch* b) {
     return TC_OK;
 }
 
-extern "C" int tc_batch_set(tc_batch* b, uint32_t n_clouds, const uint32_t* counts, const float* mins,
-                            const float* ranges, tc_stream stream) {
+extern "C" int tc_batch_set_io(tc_batch* b, uint32_t n_clouds, const uint32_t* counts, const float* mins,
+                               const float* ranges, const float* const* cloud_pts, const tc_tree_ptrs* outs,
+                               tc_stream stream) {
     if (!b || (n_clouds && !counts) || n_clouds > b->cap_clouds) return TC_ERR_INVALID;
     const uint32_t C = b->cap_clouds;
+    uint64_t pts = 0, tiles = 0;
+    for (uint32_t c = 0; c < n_clouds; ++c) {
+        pts += counts[c];
+        tiles += ceil_div_u32(counts[c], b->tile);
+        const float r = ranges ? ranges[c] : 1024.0f;
+        if (!(r != 0.0f)) return TC_ERR_INVALID;
+        if (cloud_pts && counts[c] && (!cloud_pts[c] || (reinterpret_cast<uintptr_t>(cloud_pts[c]) & 15u)))
+            return TC_ERR_INVALID;
+    }
+    if (pts > b->cap_points) return TC_ERR_CAPACITY;
+    if (tiles > b->cap_tiles) return TC_ERR_CAPACITY;
     // The previous upload must have left the staging buffer.
     TC_HIP(hipEventSynchronize(b->plan_copied));
     uint32_t* h = b->h_plan;
-    uint32_t* pt_off = h + 8;
-    uint32_t* first = pt_off + (C + 1);
-    float* mn = reinterpret_cast<float*>(first + (C + 1));
-    float* rg = mn + C;
-    uint64_t pts = 0, tiles = 0;
+    const Plan hp = plan_at(h, C);
+    uint32_t* pt_off = const_cast<uint32_t*>(hp.pt_off);
+    uint32_t* first = const_cast<uint32_t*>(hp.first_tile);
+    float* mn = const_cast<float*>(hp.min_c);
+    float* rg = const_cast<float*>(hp.range_c);
+    uint64_t* in_ptr = const_cast<uint64_t*>(hp.in_ptr);
+    tc_tree_ptrs* out = const_cast<tc_tree_ptrs*>(hp.out);
+    pts = 0;
+    tiles = 0;
     for (uint32_t c = 0; c < n_clouds; ++c) {
         pt_off[c] = (uint32_t)pts;
         first[c] = (uint32_t)tiles;
@@ -445,32 +517,38 @@ extern "C" int tc_batch_set(tc_batch* b, uint32_t n_clouds, const uint32_t* coun
         tiles += ceil_div_u32(counts[c], b->tile);
         mn[c] = mins ? mins[c] : 0.0f;
         rg[c] = ranges ? ranges[c] : 1024.0f;
-        if (!(rg[c] != 0.0f)) return TC_ERR_INVALID;
+        in_ptr[c] = cloud_pts ? reinterpret_cast<uint64_t>(cloud_pts[c]) : 0ull;
+        if (outs) out[c] = outs[c];
     }
-    if (pts > b->cap_points) return TC_ERR_CAPACITY;
-    if (tiles > b->cap_tiles) return TC_ERR_CAPACITY;
     pt_off[n_clouds] = (uint32_t)pts;
     first[n_clouds] = (uint32_t)tiles;
     h[0] = n_clouds;
     h[1] = (uint32_t)tiles;
     h[2] = (uint32_t)pts;
+    h[3] = (cloud_pts ? PLAN_IN_PTRS : 0u) | (outs ? PLAN_OUT_PTRS : 0u);
     b->n_clouds = n_clouds;
     b->n_points = (uint32_t)pts;
     b->n_tiles = (uint32_t)tiles;
+    b->in_ptrs = cloud_pts != nullptr;
     hipStream_t s = as_stream(stream);
     TC_HIP(hipMemcpyAsync(b->plan, h, b->plan_words * 4, hipMemcpyHostToDevice, s));
     TC_HIP(hipEventRecord(b->plan_copied, s));
     return TC_OK;
 }
 
+extern "C" int tc_batch_set(tc_batch* b, uint32_t n_clouds, const uint32_t* counts, const float* mins,
+                            const float* ranges, tc_stream stream) {
+    return tc_batch_set_io(b, n_clouds, counts, mins, ranges, nullptr, nullptr, stream);
+}
+
 extern "C" int tc_batch_build(tc_batch* b, const float* pts, int link_leaves, tc_stream stream) {
-    if (!b || (b->n_points && !pts)) return TC_ERR_INVALID;
+    if (!b || (b->n_points && !pts && !b->in_ptrs)) return TC_ERR_INVALID;
     if (reinterpret_cast<uintptr_t>(pts) & 15u) return TC_ERR_INVALID;
     return run_pipeline(b, pts, link_leaves, as_stream(stream));
 }
 
 extern "C" int tc_batch_capture(tc_batch* b, const float* pts, int link_leaves, tc_stream stream) {
-    if (!b || !pts || (reinterpret_cast<uintptr_t>(pts) & 15u)) return TC_ERR_INVALID;
+    if (!b || (!pts && !b->in_ptrs) || (reinterpret_cast<uintptr_t>(pts) & 15u)) return TC_ERR_INVALID;
     hipStream_t s = as_stream(stream);
     if (!s) return TC_ERR_INVALID;  // the legacy stream cannot be captured
     if (b->exec) {
@@ -502,6 +580,7 @@ extern "C" int tc_batch_capture(tc_batch* b, const float* pts, int link_leaves, 
 
 extern "C" int tc_batch_replay(tc_batch* b, tc_stream stream) {
     if (!b || !b->exec) return TC_ERR_INVALID;
+    if (!b->graph_pts && !b->in_ptrs && b->n_points) return TC_ERR_INVALID;  // captured without an input array
     ProfScope prof(b->ctx, as_stream(stream), K_SEG);
     TC_HIP(hipGraphLaunch(b->exec, as_stream(stream)));
     return TC_OK;
@@ -543,5 +622,24 @@ extern "C" int tc_batch_arrays(tc_batch* b, tc_batch_view* v) {
     v->offsets = b->offsets;
     v->nodes = b->nodes;
     v->node_capacity = b->node_cap;
+    return TC_OK;
+}
+
+extern "C" uint64_t tc_batch_memory_bytes(uint32_t max_clouds, uint32_t max_points, int code_bits) {
+    const uint64_t C = max_clouds, N = max_points ? max_points : 1;
+    const uint64_t ks = code_bits == 30 ? 4 : 8;
+    const uint64_t dig = code_bits == 30 ? 4 : 8;
+    const uint64_t tiles = ceil_div_u32(N, 256 * 8) + C;  // smallest tile shape: an upper bound
+    const uint64_t node_cap = tc_octree_node_bound(N, code_bits) + C;
+    return plan_words(max_clouds) * 4 + C * dig * 256 * 4 + 2 * tiles * 256 * 4 + 64 + 2 * (C + 1) * 4 + 16 +
+           3 * N * ks + 3 * N + 3 * N * 4 + (N + 1) * 4 + node_cap * sizeof(tc_oct_node);
+}
+
+extern "C" int tc_batch_read_info_async(tc_batch* b, uint32_t* h_unique_base, uint32_t* h_node_base,
+                                        tc_stream stream) {
+    if (!b || !h_unique_base || !h_node_base) return TC_ERR_INVALID;
+    const size_t bytes = ((size_t)b->n_clouds + 1) * 4;
+    TC_HIP(hipMemcpyAsync(h_unique_base, b->ubase, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    TC_HIP(hipMemcpyAsync(h_node_base, b->node_base, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
     return TC_OK;
 }

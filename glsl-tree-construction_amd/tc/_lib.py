@@ -92,6 +92,78 @@ class TcBatchView(C.Structure):
     ]
 
 
+class TcTreePtrs(C.Structure):
+    _fields_ = [
+        ("unique", C.c_void_p),
+        ("prefix_n", C.c_void_p),
+        ("has_leaf_left", C.c_void_p),
+        ("has_leaf_right", C.c_void_p),
+        ("left_child", C.c_void_p),
+        ("parent", C.c_void_p),
+        ("edge_count", C.c_void_p),
+        ("offsets", C.c_void_p),
+        ("nodes", C.c_void_p),
+    ]
+
+
+class TcServerConfig(C.Structure):
+    _fields_ = [
+        ("mem_budget_bytes", C.c_uint64),
+        ("n_streams", C.c_uint32),
+        ("max_batch_clouds", C.c_uint32),
+        ("max_batch_points", C.c_uint32),
+        ("code_bits", C.c_int),
+        ("link_leaves", C.c_int),
+    ]
+
+
+class TcServerResult(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint64),
+        ("status", C.c_int32),
+        ("n_points", C.c_uint32),
+        ("n_unique", C.c_uint32),
+        ("n_nodes", C.c_uint32),
+        ("tree", TcTreePtrs),
+        ("t_submit", C.c_double),
+        ("t_start", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+
+class TcServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("rejected", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("points_completed", C.c_uint64),
+        ("mean_clouds_per_batch", C.c_double),
+        ("latency_p50_ms", C.c_double),
+        ("latency_p99_ms", C.c_double),
+        ("latency_mean_ms", C.c_double),
+        ("latency_max_ms", C.c_double),
+        ("first_submit_s", C.c_double),
+        ("last_done_s", C.c_double),
+        ("reserved_bytes", C.c_uint64),
+        ("peak_reserved_bytes", C.c_uint64),
+        ("result_budget_bytes", C.c_uint64),
+    ]
+
+
+class TcSimConfig(C.Structure):
+    _fields_ = [
+        ("n_streams", C.c_uint32),
+        ("max_batch_clouds", C.c_uint32),
+        ("max_batch_points", C.c_uint32),
+        ("result_budget_bytes", C.c_uint64),
+        ("code_bits", C.c_int),
+        ("launch_s", C.c_double),
+        ("per_point_s", C.c_double),
+        ("hold_s", C.c_double),
+    ]
+
+
 P = C.c_void_p
 I = C.c_int
 U32 = C.c_uint32
@@ -161,6 +233,22 @@ SIGNATURES = {
     "tc_batch_replay": (I, [P, S]),
     "tc_batch_read_info": (I, [P, P, S]),
     "tc_batch_arrays": (I, [P, P]),
+    "tc_batch_set_io": (I, [P, U32, P, P, P, P, P, S]),
+    "tc_batch_read_info_async": (I, [P, P, P, S]),
+    "tc_batch_memory_bytes": (U64, [U32, U32, I]),
+    "tc_tree_result_bytes": (U64, [U32, I]),
+    "tc_server_config_default": (None, [C.POINTER(TcServerConfig)]),
+    "tc_server_create": (I, [P, C.POINTER(TcServerConfig), C.POINTER(C.c_void_p)]),
+    "tc_server_destroy": (I, [P]),
+    "tc_server_submit": (I, [P, P, U32, F, F, C.POINTER(U64)]),
+    "tc_server_poll": (I, [P, C.POINTER(TcServerResult), U32, C.POINTER(U32), I]),
+    "tc_server_release": (I, [P, U64]),
+    "tc_server_drain": (I, [P]),
+    "tc_server_stats_get": (I, [P, C.POINTER(TcServerStats)]),
+    "tc_server_stats_reset": (I, [P]),
+    "tc_server_run_trace": (I, [P, U32, P, P, P, F, F, P]),
+    "tc_plain_run_trace": (I, [P, U32, P, P, P, F, F, I, P]),
+    "tc_server_simulate": (I, [C.POINTER(TcSimConfig), U32, P, P, P, P, P, C.POINTER(U64)]),
     "tc_gen_uniform": (None, [P, U64, U64, F]),
     "tc_gen_gaussian_mixture": (None, [P, U64, U64, U32, F, F]),
 }

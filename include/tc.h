@@ -284,12 +284,36 @@ typedef struct {
     uint64_t node_capacity;
 } tc_batch_view;
 
+/* Per-cloud output buffers (device), sized for a cloud of n points: unique
+ * [n] codes, radix arrays [n-1], edge_count [n-1], offsets [n],
+ * nodes [tc_octree_node_bound(n)].  Filled with the same values as a plain
+ * build (cloud-local indices and offsets). */
+typedef struct {
+    void* unique;
+    uint8_t* prefix_n;
+    uint8_t* has_leaf_left;
+    uint8_t* has_leaf_right;
+    int32_t* left_child;
+    int32_t* parent;
+    uint32_t* edge_count;
+    uint32_t* offsets;
+    tc_oct_node* nodes;
+} tc_tree_ptrs;
+
 int tc_batch_create(tc_ctx* ctx, uint32_t max_clouds, uint32_t max_points, int code_bits, tc_batch** out);
 int tc_batch_destroy(tc_batch* batch);
 /* Describe the next batch: per-cloud point counts (and min/range, NULL for
  * 0 / 1024); uploads the plan asynchronously on the stream. */
 int tc_batch_set(tc_batch* batch, uint32_t n_clouds, const uint32_t* counts, const float* mins,
                  const float* ranges, tc_stream stream);
+/* As tc_batch_set, for clouds held in separate device buffers: cloud c is read
+ * from cloud_pts[c] (16-byte aligned xyzw), and when outs is non-NULL its
+ * outputs are also copied into outs[c] at the end of the build (the
+ * concatenated arrays stay valid as well).  Build or capture with
+ * pts_xyzw = NULL after this call. */
+int tc_batch_set_io(tc_batch* batch, uint32_t n_clouds, const uint32_t* counts, const float* mins,
+                    const float* ranges, const float* const* cloud_pts, const tc_tree_ptrs* outs,
+                    tc_stream stream);
 int tc_batch_build(tc_batch* batch, const float* pts_xyzw, int link_leaves, tc_stream stream);
 /* Capture the whole batched pipeline (for pts_xyzw) as a HIP graph; replay it
  * after each tc_batch_set.  The stream must not be the legacy NULL stream. */
@@ -298,6 +322,98 @@ int tc_batch_replay(tc_batch* batch, tc_stream stream);
 /* Synchronise the stream and read per-cloud offsets and counts. */
 int tc_batch_read_info(tc_batch* batch, tc_cloud_info* info, tc_stream stream);
 int tc_batch_arrays(tc_batch* batch, tc_batch_view* view);
+/* Queue copies of the per-cloud bases (n_clouds + 1 entries each: cloud c has
+ * h_unique_base[c+1] - h_unique_base[c] unique codes and
+ * h_node_base[c+1] - h_node_base[c] octree nodes) into pinned host memory;
+ * they are valid once the stream has reached this point. */
+int tc_batch_read_info_async(tc_batch* batch, uint32_t* h_unique_base, uint32_t* h_node_base, tc_stream stream);
+/* Device bytes tc_batch_create allocates for these capacities. */
+uint64_t tc_batch_memory_bytes(uint32_t max_clouds, uint32_t max_points, int code_bits);
+/* Bytes of the per-cloud output buffers (tc_tree_ptrs) for a cloud of n points,
+ * each array 256-byte aligned, nodes last, at the node bound. */
+uint64_t tc_tree_result_bytes(uint32_t n, int code_bits);
+
+/* ---- Serving (no reference counterpart).  Requests (device point buffers)
+ *      are admitted as they arrive while a device-memory budget allows, and
+ *      built in batches on several streams: whenever a stream is free, the
+ *      requests waiting at that moment (in arrival order, within the batch
+ *      limits and the budget) go out as one segmented build, so work that
+ *      arrives while other batches run overlaps them on the device.  Each
+ *      result stays in device memory, equal to a plain build of the same
+ *      points, until tc_server_release.  Submission and polling are
+ *      thread-safe; a scheduler thread owned by the server does the rest. */
+typedef struct tc_server tc_server;
+
+typedef struct {
+    uint64_t mem_budget_bytes;  /* device bytes for workspaces + held results (default 64 GiB) */
+    uint32_t n_streams;         /* concurrent batches (default 2) */
+    uint32_t max_batch_clouds;  /* per batch (default 256) */
+    uint32_t max_batch_points;  /* per batch, also the largest request (default 2^24) */
+    int code_bits;              /* 30 or 63 (default 30) */
+    int link_leaves;            /* default 1 */
+} tc_server_config;
+
+typedef struct {
+    uint64_t id;
+    int32_t status;                      /* TC_OK */
+    uint32_t n_points, n_unique, n_nodes;
+    tc_tree_ptrs tree;                   /* device arrays, valid until tc_server_release(id) */
+    double t_submit, t_start, t_done;    /* seconds on the server's steady clock */
+} tc_server_result;
+
+typedef struct {
+    uint64_t submitted, completed, rejected, batches;
+    uint64_t points_completed;
+    double mean_clouds_per_batch;
+    double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;
+    double first_submit_s, last_done_s;  /* throughput = points_completed / (last_done - first_submit) */
+    uint64_t reserved_bytes, peak_reserved_bytes, result_budget_bytes;
+} tc_server_stats;
+
+/* Fills cfg with the defaults. */
+void tc_server_config_default(tc_server_config* cfg);
+int tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server** out);
+/* Waits for the batches on the device, drops requests not yet launched, then
+ * frees everything (held results included). */
+int tc_server_destroy(tc_server* srv);
+/* Queue a build of n points (device, 16-byte aligned xyzw; must stay valid
+ * until the result is polled).  TC_ERR_CAPACITY: n above max_batch_points;
+ * TC_ERR_OVER_BUDGET: its result could never fit the budget. */
+int tc_server_submit(tc_server* srv, const float* pts_xyzw, uint32_t n, float min_coord, float range,
+                     uint64_t* id);
+/* Move up to max finished results to out (wait up to wait_ms for the first). */
+int tc_server_poll(tc_server* srv, tc_server_result* out, uint32_t max, uint32_t* n_out, int wait_ms);
+/* Return a polled result's device memory to the budget. */
+int tc_server_release(tc_server* srv, uint64_t id);
+/* Block until every submitted request has finished (requests waiting for
+ * memory need earlier results to be released meanwhile). */
+int tc_server_drain(tc_server* srv);
+int tc_server_stats_get(tc_server* srv, tc_server_stats* st);
+int tc_server_stats_reset(tc_server* srv);
+
+/* Replay an arrival trace against the server from a submitting thread:
+ * request i (points pts[i], count n[i]) is submitted at t_arrive[i] seconds
+ * after the call starts; results are released as they are polled.  latency_s
+ * receives done - arrival per request. */
+int tc_server_run_trace(tc_server* srv, uint32_t n_req, const double* t_arrive, const float* const* pts,
+                        const uint32_t* n, float min_coord, float range, double* latency_s);
+/* The same trace through a one-request-at-a-time plain loop (tc_build_tree,
+ * each request waiting for the previous one): the baseline for the server. */
+int tc_plain_run_trace(tc_ctx* ctx, uint32_t n_req, const double* t_arrive, const float* const* pts,
+                       const uint32_t* n, float min_coord, float range, int code_bits, double* latency_s);
+
+/* Discrete-event simulation of the server's admission and batching policy
+ * (the same scheduler code, a modelled device: a batch of P points takes
+ * launch_s + per_point_s * P, results are released hold_s after they finish).
+ * For testing the policy without a GPU. */
+typedef struct {
+    uint32_t n_streams, max_batch_clouds, max_batch_points;
+    uint64_t result_budget_bytes;
+    int code_bits;
+    double launch_s, per_point_s, hold_s;
+} tc_sim_config;
+int tc_server_simulate(const tc_sim_config* cfg, uint32_t n_req, const double* t_arrive, const uint32_t* n,
+                       double* t_start, double* t_done, uint32_t* batch_of, uint64_t* peak_reserved);
 
 /* ---- Host-array mirrors of the reference's per-stage functions (blocking).
  * Argument order and meaning follow the Rust functions cited; sizes are

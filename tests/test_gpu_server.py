@@ -1,0 +1,202 @@
+"""Serving on the GPU: every request's result equals a plain single-cloud build
+of the same points (bit for bit, every array), alone and in a mix of
+concurrent requests; admission limits and budget accounting."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from gpu_util import dev
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(90)]
+
+ARRAYS = ("unique", "prefix_n", "has_leaf_left", "has_leaf_right", "left_child", "parent", "edge_count", "offsets")
+
+
+def plain_build(ctx, pts, code_bits=30, rng=1024.0):
+    import tc
+
+    n = len(pts)
+    tree = tc.Tree(ctx, max(n, 1), code_bits)
+    try:
+        tree.build(dev(pts), n, 0.0, rng)
+        return tree.download()
+    finally:
+        tree.free()
+
+
+def assert_same_tree(got, want, tag=""):
+    assert got["n_unique"] == want["n_unique"], tag
+    assert got["n_nodes"] == want["n_nodes"], tag
+    for k in ARRAYS:
+        assert np.array_equal(got[k], want[k]), (tag, k)
+    assert np.array_equal(got["nodes"].view(np.uint8), want["nodes"].view(np.uint8)), (tag, "nodes")
+
+
+def make_clouds(tclib, k, seed, lo=2 ** 8, hi=2 ** 17):
+    r = np.random.default_rng(seed)
+    sizes = np.round(np.exp(r.uniform(np.log(lo), np.log(hi), k))).astype(int)
+    sizes[:4] = [0, 1, 2, 3]
+    clouds = []
+    for i, s in enumerate(sizes):
+        if i % 2:
+            clouds.append(tclib.gen_gaussian_mixture(int(s), seed * 1000 + i, 16, 4.0))
+        else:
+            clouds.append(tclib.gen_uniform(int(s), seed * 1000 + i))
+    return clouds
+
+
+def collect(srv, ids, timeout_s=60.0):
+    out = {}
+    t_end = time.time() + timeout_s
+    while len(out) < len(ids):
+        assert time.time() < t_end, "server did not finish"
+        for res in srv.poll(64, wait_ms=50):
+            out[res["id"]] = res
+    return out
+
+
+def test_served_results_equal_plain_builds(gpu_ctx, tclib):
+    import tc
+
+    clouds = make_clouds(tclib, 48, seed=7)
+    srv = tc.Server(gpu_ctx, mem_budget_bytes=16 << 30, n_streams=2, max_batch_clouds=16,
+                    max_batch_points=1 << 20)
+    try:
+        d_pts = [dev(c) for c in clouds]
+        ids = [srv.submit(p, len(c)) for p, c in zip(d_pts, clouds)]
+        got = collect(srv, ids)
+        st = srv.stats()
+        assert st["completed"] == len(clouds) and st["batches"] < len(clouds)
+        for i, (rid, c) in enumerate(zip(ids, clouds)):
+            res = got[rid]
+            assert res["status"] == 0 and res["n_points"] == len(c)
+            assert res["t_submit"] <= res["t_start"] <= res["t_done"]
+            assert_same_tree(srv.download(res), plain_build(gpu_ctx, c), f"cloud {i} n={len(c)}")
+            srv.release(rid)
+        assert srv.stats()["reserved_bytes"] == 0
+    finally:
+        srv.close()
+
+
+def test_result_same_alone_and_in_a_concurrent_mix(gpu_ctx, tclib):
+    import tc
+
+    probe = tclib.gen_gaussian_mixture(150_000, 99, 16, 4.0)
+    others = make_clouds(tclib, 40, seed=8, hi=2 ** 18)
+    srv = tc.Server(gpu_ctx, mem_budget_bytes=16 << 30, n_streams=3, max_batch_clouds=8,
+                    max_batch_points=1 << 21)
+    try:
+        d_probe = dev(probe)
+        rid = srv.submit(d_probe)
+        alone = collect(srv, [rid])[rid]
+        alone_tree = srv.download(alone)
+        srv.release(rid)
+
+        d_others = [dev(c) for c in others]
+        ids = []
+        lock = threading.Lock()
+
+        def client(lo, hi, seed):
+            r = np.random.default_rng(seed)
+            for j in range(lo, hi):
+                time.sleep(float(r.uniform(0, 2e-4)))
+                with lock:
+                    ids.append(srv.submit(d_others[j], len(others[j])))
+
+        threads = [threading.Thread(target=client, args=(k * 10, k * 10 + 10, k)) for k in range(4)]
+        for t in threads:
+            t.start()
+        time.sleep(5e-4)
+        probe_id = srv.submit(d_probe)
+        for t in threads:
+            t.join()
+        got = collect(srv, ids + [probe_id])
+        assert_same_tree(srv.download(got[probe_id]), alone_tree, "probe in mix")
+        assert_same_tree(alone_tree, plain_build(gpu_ctx, probe), "probe alone vs plain")
+        for rid in got:
+            srv.release(rid)
+    finally:
+        srv.close()
+
+
+def test_63bit_served(gpu_ctx, tclib):
+    import tc
+
+    clouds = [tclib.gen_uniform(n, 60 + n, 2097152.0) for n in (5000, 70_000, 1)]
+    srv = tc.Server(gpu_ctx, mem_budget_bytes=8 << 30, max_batch_clouds=8, max_batch_points=1 << 18, code_bits=63)
+    try:
+        d = [dev(c) for c in clouds]
+        ids = [srv.submit(p, len(c), 0.0, 2097152.0) for p, c in zip(d, clouds)]
+        got = collect(srv, ids)
+        for rid, c in zip(ids, clouds):
+            assert_same_tree(srv.download(got[rid]), plain_build(gpu_ctx, c, 63, 2097152.0), f"n={len(c)}")
+            srv.release(rid)
+    finally:
+        srv.close()
+
+
+def test_admission_limits(gpu_ctx, tclib):
+    import tc
+
+    srv = tc.Server(gpu_ctx, mem_budget_bytes=4 << 30, max_batch_clouds=4, max_batch_points=1 << 16)
+    try:
+        big = dev(tclib.gen_uniform(1 << 17, 5))
+        with pytest.raises(tc.TcError) as e:
+            srv.submit(big)
+        assert e.value.status == -6  # TC_ERR_CAPACITY
+        assert srv.stats()["rejected"] == 1
+        with pytest.raises(tc.TcError):
+            srv.release(12345)  # unknown id
+    finally:
+        srv.close()
+    # A budget that leaves no room for results cannot be created.
+    ws = int(tc.lib().tc_batch_memory_bytes(4, 1 << 16, 30))
+    with pytest.raises(tc.TcError) as e:
+        tc.Server(gpu_ctx, mem_budget_bytes=ws, n_streams=1, max_batch_clouds=4, max_batch_points=1 << 16)
+    assert e.value.status == -5  # TC_ERR_OVER_BUDGET
+
+
+def test_budget_holds_back_admission_until_release(gpu_ctx, tclib):
+    import tc
+
+    n = 1 << 16
+    ws = int(tc.lib().tc_batch_memory_bytes(4, 1 << 17, 30))
+    # Room for one worst-case result: after the first build its result is cut
+    # back to its actual size, which still leaves too little for the next.
+    srv = tc.Server(gpu_ctx, mem_budget_bytes=ws + tc.result_bytes(n) + 4096, n_streams=1, max_batch_clouds=4,
+                    max_batch_points=1 << 17)
+    try:
+        d = [dev(tclib.gen_uniform(n, 300 + i)) for i in range(3)]
+        ids = [srv.submit(p) for p in d]
+        for k in range(3):
+            got = collect(srv, ids[k:k + 1])
+            time.sleep(0.05)
+            assert srv.poll(8) == []  # the next one waits for memory
+            st = srv.stats()
+            assert 0 < st["reserved_bytes"] <= st["result_budget_bytes"]
+            srv.release(ids[k])
+            assert got[ids[k]]["status"] == 0
+        assert srv.stats()["reserved_bytes"] == 0
+    finally:
+        srv.close()
+
+
+def test_trace_runners(gpu_ctx, tclib):
+    import tc
+
+    r = np.random.default_rng(3)
+    k = 60
+    t = np.cumsum(r.exponential(1 / 2000.0, k))
+    sizes = np.round(np.exp(r.uniform(np.log(2 ** 12), np.log(2 ** 18), k))).astype(np.uint32)
+    pts = [dev(tclib.gen_uniform(int(s), 500 + i)) for i, s in enumerate(sizes)]
+    srv = tc.Server(gpu_ctx, mem_budget_bytes=8 << 30, max_batch_clouds=32, max_batch_points=1 << 21)
+    try:
+        lat = srv.run_trace(t, pts, sizes)
+        assert np.all(lat > 0) and np.all(lat < 5.0)
+        assert srv.stats()["completed"] == k
+    finally:
+        srv.close()
+    lat_p = tc.plain_run_trace(gpu_ctx, t, pts, sizes)
+    assert np.all(lat_p > 0) and np.all(lat_p < 5.0)
