@@ -18,6 +18,9 @@ Also reported (same JSON line):
   batched       1024 clouds, sizes log-uniform in [2^10, 2^20], alternating
                 uniform and 16-cluster Gaussian (sigma 4): one segmented launch
                 per stage replayed as a HIP graph vs one plain build per cloud
+  served        Poisson arrivals (1k-50k req/s) of requests sized log-uniform in
+                [2^12, 2^22]: the server vs a one-at-a-time plain loop on the
+                same traces, p50/p99 latency and points/s
   cpu_baseline  the single-threaded C port of the pipeline (the oracle) on the
                 same input, on this host
 """
@@ -222,6 +225,65 @@ def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04):
     }
 
 
+def bench_served(ctx, rates=(1000, 2000, 5000, 10000, 20000, 50000), pool=256, max_req=2000, window_s=0.5,
+                 slo_ms=10.0, seed=0xC0FFEE05):
+    """Poisson arrivals of build requests (sizes log-uniform in [2^12, 2^22],
+    uniform xyz in [0,1024)^3) against the server (64 GiB budget, 2 streams) and
+    against a one-request-at-a-time plain loop, on the same traces.  Inputs are
+    a pool of distinct clouds resident in HBM, cycled through by the trace."""
+    import numpy as np
+    import torch
+
+    import tc
+
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(2 ** 12), np.log(2 ** 22), pool))).astype(np.uint32)
+    pts = [torch.from_numpy(tc.gen_uniform(int(s), seed + i)).cuda() for i, s in enumerate(sizes)]
+    srv = tc.Server(ctx)  # defaults: 64 GiB budget, 2 streams, <= 256 clouds / 2^24 points per batch
+    # Warm up both paths.
+    warm = np.arange(8) * 1e-4
+    srv.run_trace(warm, pts[:8], sizes[:8])
+    tc.plain_run_trace(ctx, warm, pts[:8], sizes[:8])
+    out = {"workload": "Poisson arrivals, request sizes log-uniform in [2^12, 2^22] (uniform xyz in [0,1024)^3, "
+                       "30-bit codes), 64 GiB budget, 2 streams; latency = result ready - arrival",
+           "slo_p99_ms": slo_ms, "rates": []}
+    for lam in rates:
+        k = int(min(max_req, lam * window_s))
+        t = np.cumsum(rng.exponential(1.0 / lam, k))
+        t -= t[0]
+        idx = rng.integers(0, pool, k)
+        req_pts = [pts[i] for i in idx]
+        req_n = sizes[idx]
+        total = float(req_n.sum())
+        srv.reset_stats()
+        lat_s = srv.run_trace(t, req_pts, req_n)
+        st = srv.stats()
+        lat_p = tc.plain_run_trace(ctx, t, req_pts, req_n)
+        span_s = float(np.max(t + lat_s))
+        span_p = float(np.max(t + lat_p))
+        out["rates"].append({
+            "rate_req_s": lam, "requests": k, "points": int(total),
+            "served": {"p50_ms": round(float(np.percentile(lat_s, 50)) * 1e3, 3),
+                       "p99_ms": round(float(np.percentile(lat_s, 99)) * 1e3, 3),
+                       "Mpoints_per_s": round(total / span_s / 1e6, 1),
+                       "mean_clouds_per_batch": round(st["mean_clouds_per_batch"], 2)},
+            "plain": {"p50_ms": round(float(np.percentile(lat_p, 50)) * 1e3, 3),
+                      "p99_ms": round(float(np.percentile(lat_p, 99)) * 1e3, 3),
+                      "Mpoints_per_s": round(total / span_p / 1e6, 1)},
+        })
+    srv.close()
+
+    def capacity(kind):
+        ok = [r[kind]["Mpoints_per_s"] for r in out["rates"] if r[kind]["p99_ms"] <= slo_ms]
+        return max(ok) if ok else 0.0
+
+    out["served_Mpoints_per_s_at_slo"] = capacity("served")
+    out["plain_Mpoints_per_s_at_slo"] = capacity("plain")
+    out["speedup_at_slo"] = round(out["served_Mpoints_per_s_at_slo"] / out["plain_Mpoints_per_s_at_slo"], 2) \
+        if out["plain_Mpoints_per_s_at_slo"] else None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +380,8 @@ def main():
         out["sort_pairs_u64"] = bench_sort_pairs_u64(ctx, 27, 2)
         torch.cuda.empty_cache()
         out["batched"] = bench_batched(ctx, 1024, max(4, args.steps // 2))
+        torch.cuda.empty_cache()
+        out["served"] = bench_served(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(pts_h)
     ctx.close()
