@@ -68,6 +68,8 @@ struct tc_batch {
 namespace {
 
 constexpr int SEG_BLOCK = 256;
+constexpr int SEG_BRT_TILE = 1024;
+constexpr int SEG_BRT_HALO = 512;
 
 // Plan views.
 constexpr uint32_t PLAN_IN_PTRS = 1u;   // desc[3]: cloud c is read from in_ptr[c]
@@ -101,15 +103,30 @@ Plan plan_at(T* base, uint32_t C) {
 
 Plan plan_view(const tc_batch* b) { return plan_at(b->plan, b->cap_clouds); }
 
-// Largest c in [0, n-1] with off[c] <= x (the non-empty segment holding x).
-TC_DEV uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t x) {
-    uint32_t lo = 0, hi = n;
+// Largest c in [lo, n-1] with off[c] <= x (the non-empty segment holding x),
+// given off[lo] <= x.
+TC_DEV uint32_t seg_search(const uint32_t* off, uint32_t lo, uint32_t n, uint32_t x) {
+    uint32_t hi = n;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (off[mid] <= x) lo = mid;
         else hi = mid;
     }
     return lo;
+}
+
+TC_DEV uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t x) { return seg_search(off, 0, n, x); }
+
+// As seg_find, starting from c0, the segment of an index <= x (a tile's first
+// slot): a few steps forward cover the usual case of a tile inside one or two
+// clouds, a binary search the rest.
+TC_DEV uint32_t seg_find_near(const uint32_t* off, uint32_t n, uint32_t x, uint32_t c0) {
+    uint32_t c = c0;
+    for (int k = 0; k < 4; ++k) {
+        if (c + 1 < n && off[c + 1] <= x) ++c;
+        else return c;
+    }
+    return seg_search(off, c, n, x);
 }
 
 // Morton codes + per-cloud digit histograms, one sort tile at a time (the
@@ -150,9 +167,14 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
         const uint32_t end = start + tile_size < end_c ? start + tile_size : end_c;
         const float mn = p.min_c[c], rg = p.range_c[c];
         const uint32_t c0 = p.pt_off[c];
-        const float4* src = (p.desc[3] & PLAN_IN_PTRS) ? reinterpret_cast<const float4*>(p.in_ptr[c]) : pts + c0;
+        // Typed as global memory so the loads are global_load, not flat.
+        using v4 = float __attribute__((ext_vector_type(4)));
+        using gptr = const __attribute__((address_space(1))) v4*;
+        const gptr src = (p.desc[3] & PLAN_IN_PTRS) ? (gptr)reinterpret_cast<const v4*>(p.in_ptr[c])
+                                                    : (gptr)reinterpret_cast<const v4*>(pts + c0);
         for (uint32_t i = start + tid; i < end; i += SEG_BLOCK) {
-            const float4 q = src[i - c0];
+            const v4 qv = src[i - c0];
+            const float4 q = make_float4(qv.x, qv.y, qv.z, qv.w);
             K code;
             if constexpr (sizeof(K) == 4) {
                 code = d_split3_10(d_quantize(q.x, mn, rg, 1024.0f)) | (d_split3_10(d_quantize(q.y, mn, rg, 1024.0f)) << 1) |
@@ -250,15 +272,34 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(const uint32_t* __restric
                                                        uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr,
                                                        int32_t* __restrict__ left, int32_t* __restrict__ parent,
                                                        uint32_t* __restrict__ ec) {
+    // Tiles of SEG_BRT_TILE slots with the unique codes of the tile and a halo
+    // staged in LDS (a cache of the concatenated array: each cloud still reads
+    // only its own codes, by cloud-local index).
+    __shared__ K s_win[SEG_BRT_TILE + 2 * SEG_BRT_HALO];
+    __shared__ uint32_t s_c0;
     const uint32_t C = desc[0];
     const uint32_t total = ubase[C];
-    const uint32_t stride = gridDim.x * SEG_BLOCK;
-    for (uint32_t g = blockIdx.x * SEG_BLOCK + threadIdx.x; g < total; g += stride) {
-        const uint32_t c = seg_find(ubase, C, g);
-        const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
-        const uint32_t i = g - b;
-        if (i + 1 < nk) brt_node<K>(uniq + b, nk, i, prefix_n + b, hll + b, hlr + b, left + b, parent + b, ec + b);
-        else ec[g] = 0u;
+    const uint32_t tiles = ceil_div_u32(total, SEG_BRT_TILE);
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t t0 = (int64_t)t * SEG_BRT_TILE;
+        const int64_t w0 = t0 > SEG_BRT_HALO ? t0 - SEG_BRT_HALO : 0;
+        const int64_t w1 = t0 + SEG_BRT_TILE + SEG_BRT_HALO < total ? t0 + SEG_BRT_TILE + SEG_BRT_HALO : total;
+        __syncthreads();
+        if (threadIdx.x == 0) s_c0 = seg_find(ubase, C, (uint32_t)t0);
+        for (int64_t j = w0 + threadIdx.x; j < w1; j += SEG_BLOCK) s_win[j - w0] = uniq[j];
+        __syncthreads();
+        const uint32_t c0 = s_c0;
+#pragma unroll 1
+        for (int k = 0; k < SEG_BRT_TILE / SEG_BLOCK; ++k) {
+            const uint32_t g = (uint32_t)t0 + k * SEG_BLOCK + threadIdx.x;
+            if (g >= total) break;
+            const uint32_t c = seg_find_near(ubase, C, g, c0);
+            const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
+            const uint32_t i = g - b;
+            const CodeView<K> view{uniq + b, as_lds<K>(s_win), (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0)};
+            if (i + 1 < nk) brt_node<K>(view, nk, i, prefix_n + b, hll + b, hlr + b, left + b, parent + b, ec + b);
+            else ec[g] = 0u;
+        }
     }
 }
 
@@ -270,16 +311,52 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
                                                           const uint8_t* __restrict__ prefix_n, const int32_t* __restrict__ left,
                                                           const uint8_t* __restrict__ hll, const uint8_t* __restrict__ hlr,
                                                           int link_leaves, Plan p) {
+    // Tiles of slots with the links of the tile and a halo staged in LDS (see
+    // k_octree_build); the window is a cache of the concatenated arrays.
+    __shared__ int32_t s_left[SEG_BRT_TILE + 2 * SEG_BRT_HALO];
+    __shared__ uint16_t s_meta[SEG_BRT_TILE + 2 * SEG_BRT_HALO];
+    __shared__ uint32_t s_list[SEG_BRT_TILE];
+    __shared__ uint32_t s_tmp[SEG_BLOCK / 64 + 2];
+    __shared__ uint32_t s_c0;
     const uint32_t C = desc[0];
     const uint32_t total = ubase[C];
-    const uint32_t stride = gridDim.x * SEG_BLOCK;
-    for (uint32_t g = blockIdx.x * SEG_BLOCK + threadIdx.x; g < total; g += stride) {
-        const uint32_t c = seg_find(ubase, C, g);
-        const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
-        const uint32_t i = g - b;
-        if (i + 1 >= nk) continue;
-        octree_build_node<K>(nodes, node_cap, node_base[c], offsets + b, ec + b, uniq + b, prefix_n + b, left + b,
-                             hll + b, hlr + b, i, nk - 1, prefix_n[b] / 3, link_leaves, p.min_c[c], p.range_c[c]);
+    const uint32_t tiles = ceil_div_u32(total, SEG_BRT_TILE);
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t t0 = (int64_t)t * SEG_BRT_TILE;
+        const int64_t w0 = t0 > SEG_BRT_HALO ? t0 - SEG_BRT_HALO : 0;
+        const int64_t w1 = t0 + SEG_BRT_TILE + SEG_BRT_HALO < total ? t0 + SEG_BRT_TILE + SEG_BRT_HALO : total;
+        __syncthreads();
+        if (threadIdx.x == 0) s_c0 = seg_find(ubase, C, (uint32_t)t0);
+        for (int64_t j = w0 + threadIdx.x; j < w1; j += SEG_BLOCK) {
+            s_left[j - w0] = left[j];
+            s_meta[j - w0] = LinkView::pack(ec[j], hll[j], hlr[j]);
+        }
+        __syncthreads();
+        const uint32_t c0 = s_c0;
+        // Compact the slots with ec > 0 (the last slot of a cloud has ec 0).
+        uint32_t count = 0;
+#pragma unroll 1
+        for (int k = 0; k < SEG_BRT_TILE / SEG_BLOCK; ++k) {
+            const int64_t g = t0 + k * SEG_BLOCK + threadIdx.x;
+            const uint32_t flag = (g < total && (s_meta[g - w0] & 255u) != 0u) ? 1u : 0u;
+            uint32_t tot;
+            const uint32_t rank = block_exclusive_scan<SEG_BLOCK>(flag, s_tmp, &tot);
+            if (flag) s_list[count + rank] = (uint32_t)g;
+            count += tot;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (uint32_t q = threadIdx.x; q < count; q += SEG_BLOCK) {
+            const uint32_t g = s_list[q];
+            const uint32_t c = seg_find_near(ubase, C, g, c0);
+            const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
+            const uint32_t i = g - b;
+            if (i + 1 >= nk) continue;
+            const LinkView lv{left + b, hll + b, hlr + b, ec + b, as_lds<int32_t>(s_left), as_lds<uint16_t>(s_meta),
+                              (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0)};
+            octree_build_node<K>(nodes, node_cap, node_base[c], offsets + b, ec + b, uniq + b, prefix_n + b, lv, i,
+                                 nk - 1, prefix_n[b] / 3, link_leaves, p.min_c[c], p.range_c[c]);
+        }
     }
 }
 

@@ -292,7 +292,7 @@ template <> struct CodeTraits<uint32_t> {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float f = (float)d_compact3_10(code >> a);
-            f = __fdiv_rn(f, 1024.0f);
+            f = __fmul_rn(f, 0x1p-10f);  // exactly f / 1024: a power-of-two scale
             f = __fmul_rn(f, range);
             out[a] = __fadd_rn(f, min_coord);
         }
@@ -306,7 +306,7 @@ template <> struct CodeTraits<uint64_t> {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float f = (float)d_compact3_21(code >> a);
-            f = __fdiv_rn(f, 2097152.0f);
+            f = __fmul_rn(f, 0x1p-21f);  // exactly f / 2^21
             f = __fmul_rn(f, range);
             out[a] = __fadd_rn(f, min_coord);
         }

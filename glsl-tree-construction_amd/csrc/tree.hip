@@ -19,6 +19,10 @@
 namespace {
 
 constexpr int TREE_BLOCK = 256;
+constexpr int BRT_TILE = 1024;  // radix nodes per workgroup step (4 per lane)
+constexpr int BRT_HALO = 512;   // codes staged in LDS on each side of the tile
+constexpr int OCT_TILE = 1024;  // radix nodes per workgroup step of the octree kernel
+constexpr int OCT_HALO = 512;
 
 TC_DEV uint32_t resolve_nodes(uint32_t n_brt_cap, const uint32_t* d_n_keys) {
     if (!d_n_keys) return n_brt_cap;
@@ -27,15 +31,33 @@ TC_DEV uint32_t resolve_nodes(uint32_t n_brt_cap, const uint32_t* d_n_keys) {
     return m < n_brt_cap ? m : n_brt_cap;
 }
 
+// Radix tree over tiles of BRT_TILE nodes: the workgroup first copies the
+// codes of its tile plus BRT_HALO on each side into LDS, so the searches of
+// every node whose range is within about BRT_HALO / 2 keys read LDS only;
+// probes beyond the window read global memory (same values either way).
 template <typename K>
 __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
     const K* __restrict__ codes, uint32_t n_keys_cap, const uint32_t* __restrict__ d_n_keys,
     uint8_t* __restrict__ prefix_n, uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr,
     int32_t* __restrict__ left_child, int32_t* __restrict__ parent, uint32_t* __restrict__ ec) {
+    __shared__ K s_win[BRT_TILE + 2 * BRT_HALO];
     const uint32_t nb = resolve_nodes(n_keys_cap > 0 ? n_keys_cap - 1 : 0, d_n_keys);
-    const uint32_t stride = gridDim.x * TREE_BLOCK;
-    for (uint32_t i = blockIdx.x * TREE_BLOCK + threadIdx.x; i < nb; i += stride)
-        brt_node<K>(codes, (int64_t)nb + 1, i, prefix_n, hll, hlr, left_child, parent, ec);
+    const int64_t nk = (int64_t)nb + 1;
+    const uint32_t tiles = ceil_div_u32(nb, BRT_TILE);
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t t0 = (int64_t)t * BRT_TILE;
+        const int64_t w0 = t0 > BRT_HALO ? t0 - BRT_HALO : 0;
+        const int64_t w1 = t0 + BRT_TILE + BRT_HALO < nk ? t0 + BRT_TILE + BRT_HALO : nk;
+        __syncthreads();  // the previous tile's searches are done with the window
+        for (int64_t j = w0 + threadIdx.x; j < w1; j += TREE_BLOCK) s_win[j - w0] = codes[j];
+        __syncthreads();
+        const CodeView<K> view{codes, as_lds<K>(s_win), (int32_t)w0, (uint32_t)(w1 - w0)};
+#pragma unroll 1
+        for (int k = 0; k < BRT_TILE / TREE_BLOCK; ++k) {
+            const int64_t i = t0 + k * TREE_BLOCK + threadIdx.x;
+            if (i < nb) brt_node<K>(view, (uint32_t)nk, (uint32_t)i, prefix_n, hll, hlr, left_child, parent, ec);
+        }
+    }
 }
 
 __global__ __launch_bounds__(TREE_BLOCK) void k_edge_count(const uint8_t* __restrict__ prefix_n,
@@ -57,13 +79,46 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_octree_build(
     const uint32_t* __restrict__ ec, const K* __restrict__ codes, const uint8_t* __restrict__ prefix_n,
     const int32_t* __restrict__ left_child, const uint8_t* __restrict__ hll, const uint8_t* __restrict__ hlr,
     uint32_t n_brt_cap, const uint32_t* __restrict__ d_n_keys, int link_leaves, float min_coord, float range) {
+    __shared__ int32_t s_left[OCT_TILE + 2 * OCT_HALO];
+    __shared__ uint16_t s_meta[OCT_TILE + 2 * OCT_HALO];
+    __shared__ uint32_t s_list[OCT_TILE];
+    __shared__ uint32_t s_tmp[TREE_BLOCK / 64 + 2];
     const uint32_t nb = resolve_nodes(n_brt_cap, d_n_keys);
     if (nb == 0) return;
     const int root_level = prefix_n[0] / 3;
-    const uint32_t stride = gridDim.x * TREE_BLOCK;
-    for (uint32_t i = blockIdx.x * TREE_BLOCK + threadIdx.x; i < nb; i += stride)
-        octree_build_node<K>(nodes, node_cap, 0u, offsets, ec, codes, prefix_n, left_child, hll, hlr, i, nb,
-                             root_level, link_leaves, min_coord, range);
+    const uint32_t tiles = ceil_div_u32(nb, OCT_TILE);
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        // Stage the links of the tile and a halo: the walks below a node stay
+        // inside its key range, so nearly all of their reads hit LDS.
+        const int64_t t0 = (int64_t)t * OCT_TILE;
+        const int64_t w0 = t0 > OCT_HALO ? t0 - OCT_HALO : 0;
+        const int64_t w1 = t0 + OCT_TILE + OCT_HALO < nb ? t0 + OCT_TILE + OCT_HALO : nb;
+        __syncthreads();
+        for (int64_t j = w0 + threadIdx.x; j < w1; j += TREE_BLOCK) {
+            s_left[j - w0] = left_child[j];
+            s_meta[j - w0] = LinkView::pack(ec[j], hll[j], hlr[j]);
+        }
+        __syncthreads();
+        // Only nodes with ec > 0 emit anything (typically well under half):
+        // compact them so that every lane of a wave has one.
+        uint32_t count = 0;
+#pragma unroll 1
+        for (int k = 0; k < OCT_TILE / TREE_BLOCK; ++k) {
+            const int64_t i = t0 + k * TREE_BLOCK + threadIdx.x;
+            const uint32_t flag = (i < nb && (s_meta[i - w0] & 255u) != 0u) ? 1u : 0u;
+            uint32_t tot;
+            const uint32_t rank = block_exclusive_scan<TREE_BLOCK>(flag, s_tmp, &tot);
+            if (flag) s_list[count + rank] = (uint32_t)i;
+            count += tot;
+        }
+        __syncthreads();
+        const LinkView lv{left_child, hll, hlr, ec, as_lds<int32_t>(s_left), as_lds<uint16_t>(s_meta), (int32_t)w0,
+                          (uint32_t)(w1 - w0)};
+#pragma unroll 1
+        for (uint32_t q = threadIdx.x; q < count; q += TREE_BLOCK)
+            octree_build_node<K>(nodes, node_cap, 0u, offsets, ec, codes, prefix_n, lv, s_list[q], nb, root_level,
+                                 link_leaves, min_coord, range);
+    }
 }
 
 int tree_grid(tc_ctx* ctx, uint32_t n) { return tc_persistent_grid(ctx, n, TREE_BLOCK); }
@@ -76,7 +131,8 @@ int brt_impl(tc_ctx* ctx, const K* codes, uint32_t n_keys, const uint32_t* d_n_k
     if (n_keys > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
     if (n_keys < 2) return TC_OK;
     ProfScope prof(ctx, s, K_BRT);
-    hipLaunchKernelGGL(k_build_radix_tree<K>, dim3(tree_grid(ctx, n_keys - 1)), dim3(TREE_BLOCK), 0, s, codes,
+    const int grid = tc_persistent_grid(ctx, ceil_div_u32(n_keys - 1, BRT_TILE), 1);
+    hipLaunchKernelGGL(k_build_radix_tree<K>, dim3(grid), dim3(TREE_BLOCK), 0, s, codes,
                        n_keys, d_n_keys, prefix_n, hll, hlr, left, parent, ec);
     TC_LAUNCH_CHECK();
     return TC_OK;
@@ -93,7 +149,7 @@ int octree_impl(tc_ctx* ctx, tc_oct_node* nodes, uint32_t cap, const uint32_t* o
     if (!nodes || !offsets || !ec || !codes || !prefix_n || !parent) return TC_ERR_INVALID;
     if (!left || !hll || !hlr) return TC_ERR_INVALID;
     if (reinterpret_cast<uintptr_t>(nodes) & 15u) return TC_ERR_INVALID;
-    const int grid = tree_grid(ctx, n_brt);
+    const int grid = tc_persistent_grid(ctx, ceil_div_u32(n_brt, OCT_TILE), 1);
     ProfScope prof(ctx, s, K_OCT_EMIT);
     hipLaunchKernelGGL(k_octree_build<K>, dim3(grid), dim3(TREE_BLOCK), 0, s, nodes, cap, offsets, ec, codes,
                        prefix_n, left, hll, hlr, n_brt, d_n_keys, link_leaves, min_coord, range);

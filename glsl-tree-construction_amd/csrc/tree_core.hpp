@@ -8,14 +8,71 @@
 
 #include "tc_internal.hpp"
 
+// Codes of one cloud, read through an optional LDS copy of c[w0, w0 + wlen)
+// (the window around a workgroup's nodes, where nearly every search probe
+// lands).  The window pointer is typed as LDS (address space 3) so that window
+// reads compile to ds_read and the rest to global loads, not both to flat
+// loads.  Indices are 32-bit: a cloud has at most 2^30 keys.
+template <typename K>
+using lds_ptr = const __attribute__((address_space(3))) K*;
+
+template <typename K>
+struct CodeView {
+    const K* __restrict__ c;
+    lds_ptr<K> win = nullptr;
+    int32_t w0 = 0;
+    uint32_t wlen = 0;
+    TC_DEV K operator[](uint32_t j) const {
+        const uint32_t r = j - (uint32_t)w0;
+        if (r < wlen) return win[r];
+        return c[j];
+    }
+};
+
+// LDS view of a __shared__ array.
+template <typename K>
+TC_DEV lds_ptr<K> as_lds(const K* p) {
+    return (lds_ptr<K>)p;
+}
+
+// Radix-tree links of one cloud (cloud-local indices), read through an
+// optional LDS copy of the window [w0, w0 + wlen): the left child and a packed
+// word edge_count | has_leaf_left << 8 | has_leaf_right << 9 (edge counts are
+// at most 21).  The octree's walks down the radix tree read these.
+struct LinkView {
+    const int32_t* __restrict__ left;
+    const uint8_t* __restrict__ hll;
+    const uint8_t* __restrict__ hlr;
+    const uint32_t* __restrict__ ec;
+    lds_ptr<int32_t> wleft = nullptr;
+    lds_ptr<uint16_t> wmeta = nullptr;
+    int32_t w0 = 0;
+    uint32_t wlen = 0;
+
+    static TC_DEV uint16_t pack(uint32_t e, uint8_t l, uint8_t r) {
+        return (uint16_t)((e < 255u ? e : 255u) | ((uint32_t)(l != 0) << 8) | ((uint32_t)(r != 0) << 9));
+    }
+    TC_DEV int32_t left_child(uint32_t x) const {
+        const uint32_t r = x - (uint32_t)w0;
+        if (r < wlen) return wleft[r];
+        return left[x];
+    }
+    TC_DEV uint32_t meta(uint32_t x) const {
+        const uint32_t r = x - (uint32_t)w0;
+        if (r < wlen) return wmeta[r];
+        return pack(ec[x], hll[x], hlr[x]);
+    }
+};
+
+// Common-prefix length of codes[i] and codes[j] over the code width; -1 for
+// j outside [0, nk) (j is taken modulo 2^32, so i - x for x > i is outside).
 template <typename K>
 struct Delta {
-    const K* __restrict__ c;
-    int64_t nk;
+    CodeView<K> c;
+    uint32_t nk;
     K ci;
-    // Common-prefix length over the code width, -1 outside [0, nk-1].
-    TC_DEV int operator()(int64_t j) const {
-        if (j < 0 || j > nk - 1) return -1;
+    TC_DEV int operator()(uint32_t j) const {
+        if (j >= nk) return -1;
         return CodeTraits<K>::delta_bits(ci ^ c[j]);
     }
 };
@@ -34,60 +91,52 @@ TC_DEV int log2_ceil_u32(uint32_t x) {
 // prefix_n[child]/3 - prefix_n[i]/3 is known here without reading the child's
 // output; node 0 writes the root's count, 1.
 template <typename K>
-TC_DEV void brt_node(const K* __restrict__ codes, int64_t nk, int64_t i, uint8_t* __restrict__ prefix_n,
+TC_DEV void brt_node(CodeView<K> codes, uint32_t nk, uint32_t i, uint8_t* __restrict__ prefix_n,
                      uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr, int32_t* __restrict__ left_child,
                      int32_t* __restrict__ parent, uint32_t* __restrict__ ec = nullptr) {
-    Delta<K> delta{codes, nk, codes[i]};
+    const Delta<K> delta{codes, nk, codes[i]};
     // Direction (Karras 2012 eq. 1); ties (duplicate codes only) take +1.
-    const int d = (delta(i + 1) - delta(i - 1)) >= 0 ? 1 : -1;
-    const int dmin = delta(i - d);
-    int64_t lmax = 2;
-    while (delta(i + lmax * d) > dmin) lmax *= 2;
-    int64_t l = 0;
-    for (int64_t t = lmax / 2; t >= 1; t /= 2)
-        if (delta(i + (l + t) * d) > dmin) l += t;
-    const int64_t j = i + l * d;
+    const bool fwd = delta(i + 1) - delta(i - 1) >= 0;
+    // Index i + d*x, modulo 2^32 (x < 2^31, nk <= 2^30: wrapped values are >= nk).
+    auto at = [&](uint32_t x) { return fwd ? i + x : i - x; };
+    const int dmin = delta(fwd ? i - 1 : i + 1);
+    uint32_t lmax = 2;
+    while (lmax <= nk && delta(at(lmax)) > dmin) lmax *= 2;
+    uint32_t l = 0;
+    for (uint32_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(at(l + t)) > dmin) l += t;
+    const uint32_t j = at(l);
     const int dnode = delta(j);
     prefix_n[i] = (uint8_t)dnode;
-    int64_t s = 0;
-    const uint32_t maxdiv = l <= 1 ? 1u : (uint32_t)1 << log2_ceil_u32((uint32_t)l);
-    for (uint32_t div = 2, t = ((uint32_t)l + 1) / 2; div <= maxdiv; div <<= 1, t = ((uint32_t)l + div - 1) / div)
-        if (delta(i + (s + (int64_t)t) * d) > dnode) s += t;
-    const int64_t gamma = i + s * d + (d < 0 ? d : 0);
+    // Split: binary search with ceil(l / 2^k) steps (shader/build_radix_tree.comp:119-129).
+    uint32_t s = 0;
+    const int kmax = l <= 1 ? 0 : log2_ceil_u32(l);
+    for (int k = 1; k <= kmax; ++k) {
+        const uint32_t t = (l + (1u << k) - 1) >> k;
+        if (delta(at(s + t)) > dnode) s += t;
+    }
+    const uint32_t gamma = fwd ? i + s : i - s - 1;
     left_child[i] = (int32_t)gamma;
-    const bool leaf_l = (i < j ? i : j) == gamma;
-    const bool leaf_r = (i > j ? i : j) == gamma + 1;
+    const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+    const bool leaf_l = lo == gamma;
+    const bool leaf_r = hi == gamma + 1;
     hll[i] = leaf_l;
     hlr[i] = leaf_r;
     if (!leaf_l) parent[gamma] = (int32_t)i;
     if (!leaf_r) parent[gamma + 1] = (int32_t)i;
     if (i == 0) parent[0] = 0;  // the reference's zero-initialised slot (src/main.rs:103)
     if (ec) {
-        const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
         const int level = dnode / 3;
         if (!leaf_l) {
-            Delta<K> dl{codes, nk, codes[lo]};
+            const Delta<K> dl{codes, nk, codes[lo]};
             ec[gamma] = (uint32_t)(dl(gamma) / 3 - level);
         }
         if (!leaf_r) {
-            Delta<K> dr{codes, nk, codes[gamma + 1]};
+            const Delta<K> dr{codes, nk, codes[gamma + 1]};
             ec[gamma + 1] = (uint32_t)(dr(hi) / 3 - level);
         }
         if (i == 0) ec[0] = 1u;
     }
-}
-
-// Nearest radix ancestor-or-self with a non-zero edge count (at most three
-// steps in a valid tree: along a run of zero counts the prefix length falls by
-// one per step while its level stays the same; the root's count is 1).
-// Bounded and range-checked so malformed input cannot loop or fault.
-TC_DEV uint32_t contributing(uint32_t i, uint32_t nb, const uint32_t* __restrict__ ec,
-                             const int32_t* __restrict__ parent) {
-    for (int step = 0; step < 64 && ec[i] == 0; ++step) {
-        const uint32_t p = (uint32_t)parent[i];
-        i = p < nb ? p : 0u;
-    }
-    return i;
 }
 
 template <typename K>
@@ -100,22 +149,24 @@ TC_DEV uint32_t child_slot(K code, int level) {
 // shader/octree.comp:126-222), written without atomics: radix node a with
 // ec[a] > 0 owns the chain offsets[a] .. offsets[a]+ec[a]-1 (bottom, i.e.
 // deepest, first).  Chain node j > 0 has one child, node j-1.  The bottom node's
-// children are found by a bounded walk down the radix tree from a through
-// nodes with ec == 0 (they share a's level): each internal node met with
-// ec > 0 contributes its chain top, each radix leaf met contributes itself
-// (SetLeaf: children[slot] = leaf, leaf mask left as cleared, i.e. 0).  Every
-// child of the bottom node sits one level below it, so its slot is the next 3
-// bits of its code.  The walk replaces the upward ancestor search of
-// shader/octree.comp:157-165 and the racy mask updates (each node is written
-// once, whole, by one lane).  offsets[] hold global node indices; stored links
-// are local (minus node_base).
+// children are found by walking down the radix tree from a through nodes with
+// ec == 0 (they share a's octree level L): each internal node met with ec > 0
+// contributes its chain top, each radix leaf met contributes itself (SetLeaf:
+// children[slot] = leaf, leaf mask left as cleared, i.e. 0).  Every child of the
+// bottom node sits one level below it, so its slot is the next 3 bits of its
+// code.  Prefix lengths of same-level nodes lie in [3L, 3L+2] and grow by at
+// least one per radix level, so the walk has at most three levels (a, two
+// nodes, four nodes) and at most eight results; it is unrolled on those
+// positions (position = path bits), which keeps every index in a register.
+// The walk replaces the upward ancestor search of shader/octree.comp:157-165
+// and the racy mask updates (each node is written once, whole, by one lane).
+// offsets[] hold global node indices; stored links are local (minus node_base).
 template <typename K>
 TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap, uint32_t node_base,
                               const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ec,
                               const K* __restrict__ codes, const uint8_t* __restrict__ prefix_n,
-                              const int32_t* __restrict__ left_child, const uint8_t* __restrict__ hll,
-                              const uint8_t* __restrict__ hlr, uint32_t a, uint32_t nb, int root_level,
-                              int link_leaves, float min_coord, float range) {
+                              const LinkView& lv, uint32_t a, uint32_t nb, int root_level, int link_leaves,
+                              float min_coord, float range) {
     constexpr int BITS = CodeTraits<K>::BITS;
     const uint32_t cnt = ec[a];
     if (cnt == 0) return;
@@ -123,37 +174,66 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
     const K code = codes[a];
     const int bottom = prefix_n[a] / 3;
 
-    // Children of the bottom node.
-    int32_t bch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int32_t bmask = 0;
-    {
-        uint32_t stack[8];
-        int sp = 0;
-        stack[sp++] = a;
-        int visits = 0;
-        while (sp > 0 && visits < 16) {
-            const uint32_t x = stack[--sp];
-            ++visits;
-            const int32_t lc = left_child[x];
-            const uint8_t leaf[2] = {hll[x], hlr[x]};
+    // Walk (LDS reads only while inside the window).  kind: 0 none,
+    // 1 radix leaf, 2 internal node with its own chain.
+    uint32_t cand[8];
+    uint32_t kind[8];
+    uint32_t mid[2];
+    bool mid_ok[2] = {false, false};
+    uint32_t low[4];
+    bool low_ok[4] = {false, false, false, false};
 #pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                const uint32_t c = (uint32_t)lc + side;
-                if (c > nb) continue;  // a valid tree's children are in [0, nb]
-                if (leaf[side]) {
-                    if (link_leaves) bch[child_slot<K>(codes[c], bottom + 1)] = (int32_t)c;
-                } else if (c < nb) {
-                    const uint32_t e = ec[c];
-                    if (e > 0) {
-                        const uint32_t slot = child_slot<K>(codes[c], bottom + 1);
-                        bch[slot] = (int32_t)(offsets[c] + e - 1 - node_base);
-                        bmask |= 1 << slot;
-                    } else if (sp < 8) {
-                        stack[sp++] = c;
-                    }
+    for (int p = 0; p < 8; ++p) {
+        cand[p] = 0;
+        kind[p] = 0;
+    }
+    auto visit = [&](uint32_t x, int pos, int stride, uint32_t* next, bool* next_ok) {
+        const uint32_t lc = (uint32_t)lv.left_child(x);
+        const uint32_t m = lv.meta(x);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t c = lc + side;
+            const int ps = pos + side * stride;
+            if (c > nb) continue;  // a valid tree's children are in [0, nb]
+            if ((m >> (8 + side)) & 1u) {
+                cand[ps] = c;
+                kind[ps] = link_leaves ? 1u : 0u;
+            } else if (c < nb) {
+                const uint32_t e = lv.meta(c) & 255u;
+                if (e > 0 || !next) {
+                    cand[ps] = c;
+                    kind[ps] = e > 0 ? 2u : 0u;
+                } else {
+                    next[side] = c;
+                    next_ok[side] = true;
                 }
             }
         }
+    };
+    visit(a, 0, 4, mid, mid_ok);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        if (mid_ok[k]) visit(mid[k], 4 * k, 2, low + 2 * k, low_ok + 2 * k);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (low_ok[k]) visit(low[k], 2 * k, 1, nullptr, nullptr);
+
+    // Children of the bottom node: slot from the child's code, link = leaf
+    // index or the child's chain top.
+    int32_t bch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t bmask = 0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        if (kind[p] == 0) continue;
+        const uint32_t c = cand[p];
+        const uint32_t slot = child_slot<K>(codes[c], bottom + 1);
+        int32_t v = (int32_t)c;
+        if (kind[p] == 2) {
+            v = (int32_t)(offsets[c] + (lv.meta(c) & 255u) - 1 - node_base);
+            bmask |= 1 << slot;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) bch[q] = (slot == (uint32_t)q) ? v : bch[q];
     }
 
     for (uint32_t j = 0; j < cnt; ++j) {
@@ -163,16 +243,19 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
         const int sh = BITS - 3 * level;
         float c4[4];
         CodeTraits<K>::decode((code >> sh) << sh, min_coord, range, c4);
-        const float cell = __fdiv_rn(range, (float)(1u << (level - root_level)));
+        // range / 2^(level - root_level), exact: a power-of-two scale.
+        const float cell = ldexpf(range, root_level - level);
         int32_t ch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         int32_t mask = 0;
         if (j == 0) {
 #pragma unroll
-            for (int s = 0; s < 8; ++s) ch[s] = bch[s];
+            for (int q = 0; q < 8; ++q) ch[q] = bch[q];
             mask = bmask;
         } else {  // chain-internal child: the node one level deeper
             const uint32_t slot = child_slot<K>(code, level + 1);
-            ch[slot] = (int32_t)(idx - 1 - node_base);
+            const int32_t v = (int32_t)(idx - 1 - node_base);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) ch[q] = (slot == (uint32_t)q) ? v : 0;
             mask = 1 << slot;
         }
         int4* dst = reinterpret_cast<int4*>(nodes + idx);
