@@ -68,8 +68,10 @@ struct tc_batch {
 namespace {
 
 constexpr int SEG_BLOCK = 256;
-constexpr int SEG_BRT_TILE = 1024;
-constexpr int SEG_BRT_HALO = 512;
+constexpr int SEG_BRT_TILE = 1024;  // slots per workgroup step of the tree kernels
+constexpr int SEG_BRT_HALO = 512;   // LDS window halo of the octree walk
+constexpr int SEG_SEARCH_HALO = 128;     // LDS window halo of the radix-tree searches
+constexpr uint32_t SEG_BRT_LIMIT = 64;  // radix ranges searched in LDS only (within SEG_SEARCH_HALO)
 
 // Plan views.
 constexpr uint32_t PLAN_IN_PTRS = 1u;   // desc[3]: cloud c is read from in_ptr[c]
@@ -274,18 +276,24 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(const uint32_t* __restric
                                                        uint32_t* __restrict__ ec) {
     // Tiles of SEG_BRT_TILE slots with the unique codes of the tile and a halo
     // staged in LDS (a cache of the concatenated array: each cloud still reads
-    // only its own codes, by cloud-local index).
-    __shared__ K s_win[SEG_BRT_TILE + 2 * SEG_BRT_HALO];
+    // only its own codes, by cloud-local index).  Bounded searches first, the
+    // long ranges afterwards (see k_build_radix_tree).
+    __shared__ K s_win[SEG_BRT_TILE + 2 * SEG_SEARCH_HALO];
+    __shared__ uint32_t s_long[SEG_BRT_TILE];
+    __shared__ uint32_t s_nlong;
     __shared__ uint32_t s_c0;
     const uint32_t C = desc[0];
     const uint32_t total = ubase[C];
     const uint32_t tiles = ceil_div_u32(total, SEG_BRT_TILE);
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const int64_t t0 = (int64_t)t * SEG_BRT_TILE;
-        const int64_t w0 = t0 > SEG_BRT_HALO ? t0 - SEG_BRT_HALO : 0;
-        const int64_t w1 = t0 + SEG_BRT_TILE + SEG_BRT_HALO < total ? t0 + SEG_BRT_TILE + SEG_BRT_HALO : total;
+        const int64_t w0 = t0 > SEG_SEARCH_HALO ? t0 - SEG_SEARCH_HALO : 0;
+        const int64_t w1 = t0 + SEG_BRT_TILE + SEG_SEARCH_HALO < total ? t0 + SEG_BRT_TILE + SEG_SEARCH_HALO : total;
         __syncthreads();
-        if (threadIdx.x == 0) s_c0 = seg_find(ubase, C, (uint32_t)t0);
+        if (threadIdx.x == 0) {
+            s_c0 = seg_find(ubase, C, (uint32_t)t0);
+            s_nlong = 0;
+        }
         for (int64_t j = w0 + threadIdx.x; j < w1; j += SEG_BLOCK) s_win[j - w0] = uniq[j];
         __syncthreads();
         const uint32_t c0 = s_c0;
@@ -296,9 +304,23 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(const uint32_t* __restric
             const uint32_t c = seg_find_near(ubase, C, g, c0);
             const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
             const uint32_t i = g - b;
+            if (i + 1 < nk) {
+                if (!brt_node_near<K, SEG_BRT_LIMIT, true>(as_lds<K>(s_win), (int32_t)(w0 - (int64_t)b), nk, i,
+                                                           prefix_n + b, hll + b, hlr + b, left + b, parent + b,
+                                                           ec + b))
+                    s_long[atomicAdd(&s_nlong, 1u)] = g;
+            } else {
+                ec[g] = 0u;
+            }
+        }
+        __syncthreads();
+        const uint32_t nlong = s_nlong;
+        for (uint32_t q = threadIdx.x; q < nlong; q += SEG_BLOCK) {
+            const uint32_t g = s_long[q];
+            const uint32_t c = seg_find_near(ubase, C, g, c0);
+            const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
             const CodeView<K> view{uniq + b, as_lds<K>(s_win), (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0)};
-            if (i + 1 < nk) brt_node<K>(view, nk, i, prefix_n + b, hll + b, hlr + b, left + b, parent + b, ec + b);
-            else ec[g] = 0u;
+            brt_node<K>(view, nk, g - b, prefix_n + b, hll + b, hlr + b, left + b, parent + b, ec + b);
         }
     }
 }

@@ -20,7 +20,8 @@ namespace {
 
 constexpr int TREE_BLOCK = 256;
 constexpr int BRT_TILE = 1024;  // radix nodes per workgroup step (4 per lane)
-constexpr int BRT_HALO = 512;   // codes staged in LDS on each side of the tile
+constexpr int BRT_HALO = 128;   // codes staged in LDS on each side of the tile
+constexpr uint32_t BRT_LIMIT = 64;  // ranges up to this many keys take the LDS-only search
 constexpr int OCT_TILE = 1024;  // radix nodes per workgroup step of the octree kernel
 constexpr int OCT_HALO = 512;
 
@@ -31,32 +32,50 @@ TC_DEV uint32_t resolve_nodes(uint32_t n_brt_cap, const uint32_t* d_n_keys) {
     return m < n_brt_cap ? m : n_brt_cap;
 }
 
-// Radix tree over tiles of BRT_TILE nodes: the workgroup first copies the
-// codes of its tile plus BRT_HALO on each side into LDS, so the searches of
-// every node whose range is within about BRT_HALO / 2 keys read LDS only;
-// probes beyond the window read global memory (same values either way).
+// Radix tree over tiles of BRT_TILE nodes.  The workgroup copies the codes of
+// its tile plus BRT_HALO on each side into LDS; every node first runs the
+// search bounded to ranges below BRT_LIMIT keys, whose probes all fall inside
+// the window (LDS reads only, short loops).  The few nodes with longer ranges
+// are collected and searched in full afterwards, by the whole workgroup.
 template <typename K>
 __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
     const K* __restrict__ codes, uint32_t n_keys_cap, const uint32_t* __restrict__ d_n_keys,
     uint8_t* __restrict__ prefix_n, uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr,
     int32_t* __restrict__ left_child, int32_t* __restrict__ parent, uint32_t* __restrict__ ec) {
+    static_assert(BRT_HALO >= BRT_LIMIT, "bounded searches must stay inside the window");
     __shared__ K s_win[BRT_TILE + 2 * BRT_HALO];
+    __shared__ uint32_t s_long[BRT_TILE];
+    __shared__ uint32_t s_nlong;
     const uint32_t nb = resolve_nodes(n_keys_cap > 0 ? n_keys_cap - 1 : 0, d_n_keys);
     const int64_t nk = (int64_t)nb + 1;
     const uint32_t tiles = ceil_div_u32(nb, BRT_TILE);
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        // Window [ws, ws + BRT_TILE + 2 BRT_HALO); slots outside [0, nk) hold
+        // the all-ones sentinel ("no common prefix").
         const int64_t t0 = (int64_t)t * BRT_TILE;
-        const int64_t w0 = t0 > BRT_HALO ? t0 - BRT_HALO : 0;
-        const int64_t w1 = t0 + BRT_TILE + BRT_HALO < nk ? t0 + BRT_TILE + BRT_HALO : nk;
-        __syncthreads();  // the previous tile's searches are done with the window
-        for (int64_t j = w0 + threadIdx.x; j < w1; j += TREE_BLOCK) s_win[j - w0] = codes[j];
+        const int64_t ws = t0 - BRT_HALO;
+        __syncthreads();  // the previous tile is done with the window
+        if (threadIdx.x == 0) s_nlong = 0;
+        for (int q = threadIdx.x; q < BRT_TILE + 2 * BRT_HALO; q += TREE_BLOCK) {
+            const int64_t j = ws + q;
+            s_win[q] = (j >= 0 && j < nk) ? codes[j] : (K)~(K)0;
+        }
         __syncthreads();
-        const CodeView<K> view{codes, as_lds<K>(s_win), (int32_t)w0, (uint32_t)(w1 - w0)};
 #pragma unroll 1
         for (int k = 0; k < BRT_TILE / TREE_BLOCK; ++k) {
             const int64_t i = t0 + k * TREE_BLOCK + threadIdx.x;
-            if (i < nb) brt_node<K>(view, (uint32_t)nk, (uint32_t)i, prefix_n, hll, hlr, left_child, parent, ec);
+            if (i < nb && !brt_node_near<K, BRT_LIMIT, false>(as_lds<K>(s_win), (int32_t)ws, (uint32_t)nk,
+                                                              (uint32_t)i, prefix_n, hll, hlr, left_child, parent,
+                                                              ec))
+                s_long[atomicAdd(&s_nlong, 1u)] = (uint32_t)i;
         }
+        __syncthreads();
+        const int64_t w0 = ws > 0 ? ws : 0;
+        const int64_t w1 = ws + BRT_TILE + 2 * BRT_HALO < nk ? ws + BRT_TILE + 2 * BRT_HALO : nk;
+        const CodeView<K> view{codes, as_lds<K>(s_win + (w0 - ws)), (int32_t)w0, (uint32_t)(w1 - w0)};
+        const uint32_t nlong = s_nlong;
+        for (uint32_t q = threadIdx.x; q < nlong; q += TREE_BLOCK)
+            brt_node<K>(view, (uint32_t)nk, s_long[q], prefix_n, hll, hlr, left_child, parent, ec);
     }
 }
 

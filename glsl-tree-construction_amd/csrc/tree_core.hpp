@@ -66,9 +66,9 @@ struct LinkView {
 
 // Common-prefix length of codes[i] and codes[j] over the code width; -1 for
 // j outside [0, nk) (j is taken modulo 2^32, so i - x for x > i is outside).
-template <typename K>
+template <typename K, typename View = CodeView<K>>
 struct Delta {
-    CodeView<K> c;
+    View c;
     uint32_t nk;
     K ci;
     TC_DEV int operator()(uint32_t j) const {
@@ -90,18 +90,27 @@ TC_DEV int log2_ceil_u32(uint32_t x) {
 // [lo, hi] has prefix length lcp(codes[lo], codes[hi]), so its count
 // prefix_n[child]/3 - prefix_n[i]/3 is known here without reading the child's
 // output; node 0 writes the root's count, 1.
-template <typename K>
-TC_DEV void brt_node(CodeView<K> codes, uint32_t nk, uint32_t i, uint8_t* __restrict__ prefix_n,
+//
+// With LIMIT > 0 the node gives up (returns false, writes nothing) when its
+// range reaches LIMIT keys: every probe then stays within LIMIT of i, so a
+// caller can serve them from an LDS window and handle the few long ranges
+// (nodes near the root) separately, instead of letting one long search hold
+// up a whole wave.
+template <typename K, uint32_t LIMIT = 0, typename View = CodeView<K>>
+TC_DEV bool brt_node(View codes, uint32_t nk, uint32_t i, uint8_t* __restrict__ prefix_n,
                      uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr, int32_t* __restrict__ left_child,
                      int32_t* __restrict__ parent, uint32_t* __restrict__ ec = nullptr) {
-    const Delta<K> delta{codes, nk, codes[i]};
+    const Delta<K, View> delta{codes, nk, codes[i]};
     // Direction (Karras 2012 eq. 1); ties (duplicate codes only) take +1.
     const bool fwd = delta(i + 1) - delta(i - 1) >= 0;
     // Index i + d*x, modulo 2^32 (x < 2^31, nk <= 2^30: wrapped values are >= nk).
     auto at = [&](uint32_t x) { return fwd ? i + x : i - x; };
     const int dmin = delta(fwd ? i - 1 : i + 1);
     uint32_t lmax = 2;
-    while (lmax <= nk && delta(at(lmax)) > dmin) lmax *= 2;
+    while (lmax <= nk && delta(at(lmax)) > dmin) {
+        lmax *= 2;
+        if (LIMIT && lmax > LIMIT) return false;
+    }
     uint32_t l = 0;
     for (uint32_t t = lmax / 2; t >= 1; t /= 2)
         if (delta(at(l + t)) > dmin) l += t;
@@ -128,15 +137,85 @@ TC_DEV void brt_node(CodeView<K> codes, uint32_t nk, uint32_t i, uint8_t* __rest
     if (ec) {
         const int level = dnode / 3;
         if (!leaf_l) {
-            const Delta<K> dl{codes, nk, codes[lo]};
+            const Delta<K, View> dl{codes, nk, codes[lo]};
             ec[gamma] = (uint32_t)(dl(gamma) / 3 - level);
         }
         if (!leaf_r) {
-            const Delta<K> dr{codes, nk, codes[gamma + 1]};
+            const Delta<K, View> dr{codes, nk, codes[gamma + 1]};
             ec[gamma + 1] = (uint32_t)(dr(hi) / 3 - level);
         }
         if (i == 0) ec[0] = 1u;
     }
+    return true;
+}
+
+// The radix-tree node of brt_node for ranges below LIMIT keys, with every
+// probe in an LDS window (w[j - w0] = codes[j] for |j - i| <= LIMIT).  Each
+// probe "delta(j) > t" is one XOR and one compare against a threshold: for
+// codes of B bits in a word of W bits, clz(x) - (W - B) > t exactly when
+// x < 2^(B - 1 - t).  With CHECK the probe also tests j < nk (windows shared by
+// several clouds); without it, window slots outside [0, nk) must hold an
+// all-ones sentinel, whose XOR with any code is at least 2^B, i.e. "no common
+// prefix".  Returns false, writing nothing, when the range reaches LIMIT.
+template <typename K, uint32_t LIMIT, bool CHECK>
+TC_DEV bool brt_node_near(lds_ptr<K> w, int32_t w0, uint32_t nk, uint32_t i, uint8_t* __restrict__ prefix_n,
+                          uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr, int32_t* __restrict__ left_child,
+                          int32_t* __restrict__ parent, uint32_t* __restrict__ ec) {
+    constexpr int B = CodeTraits<K>::BITS;
+    auto code = [&](uint32_t j) { return w[(int32_t)j - w0]; };
+    auto dlt = [&](K a, K b) {  // -1 for "outside" (sentinel)
+        const int d = CodeTraits<K>::delta_bits(a ^ b);
+        return d > -1 ? d : -1;
+    };
+    auto thr = [&](int t) { return t >= B ? (K)0 : (K)((K)1 << (B - 1 - t)); };  // delta > t  <=>  x < thr(t)
+    const K ci = code(i);
+    // i >= 1 or the sentinel below 0 (CHECK: i - 1 wraps to >= nk).
+    const int dl = (!CHECK || i >= 1) ? dlt(ci, code(i - 1)) : -1;
+    const int dr = dlt(ci, code(i + 1));  // i + 1 <= nk - 1: always a key
+    const bool fwd = dr >= dl;
+    auto at = [&](uint32_t x) { return fwd ? i + x : i - x; };
+    auto probe = [&](uint32_t x, K t) {
+        const uint32_t j = at(x);
+        if (CHECK && j >= nk) return false;
+        return (K)(ci ^ code(j)) < t;
+    };
+    const int dmin = fwd ? dl : dr;
+    const K tmin = thr(dmin);
+    uint32_t lmax = 2;
+    while (probe(lmax, tmin)) {
+        lmax *= 2;
+        if (lmax > LIMIT) return false;
+    }
+    uint32_t l = 0;
+    for (uint32_t t = lmax / 2; t >= 1; t /= 2)
+        if (probe(l + t, tmin)) l += t;
+    const uint32_t j = at(l);
+    const int dnode = dlt(ci, code(j));
+    prefix_n[i] = (uint8_t)dnode;
+    const K tnode = thr(dnode);
+    uint32_t s = 0;
+    const int kmax = l <= 1 ? 0 : log2_ceil_u32(l);
+    for (int k = 1; k <= kmax; ++k) {
+        const uint32_t t = (l + (1u << k) - 1) >> k;
+        if (probe(s + t, tnode)) s += t;
+    }
+    const uint32_t gamma = fwd ? i + s : i - s - 1;
+    left_child[i] = (int32_t)gamma;
+    const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+    const bool leaf_l = lo == gamma;
+    const bool leaf_r = hi == gamma + 1;
+    hll[i] = leaf_l;
+    hlr[i] = leaf_r;
+    if (!leaf_l) parent[gamma] = (int32_t)i;
+    if (!leaf_r) parent[gamma + 1] = (int32_t)i;
+    if (i == 0) parent[0] = 0;
+    if (ec) {
+        const int level = dnode / 3;
+        if (!leaf_l) ec[gamma] = (uint32_t)(dlt(code(lo), code(gamma)) / 3 - level);
+        if (!leaf_r) ec[gamma + 1] = (uint32_t)(dlt(code(gamma + 1), code(hi)) / 3 - level);
+        if (i == 0) ec[0] = 1u;
+    }
+    return true;
 }
 
 template <typename K>
