@@ -294,6 +294,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline build only")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true", help="fixed sort launch shape instead of the autotuner's")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -307,6 +308,7 @@ def main():
     n = 1 << args.log2n
     pts_h = tc.gen_gaussian_mixture(n, args.seed, 64, 8.0, 1024.0)
     ctx = tc.Context(0)
+    tuning = None if args.no_tune else tc.tune.autotune(ctx, n)
     pts = torch.from_numpy(pts_h).cuda()
     tree = tc.Tree(ctx, n, 30)
 
@@ -368,6 +370,8 @@ def main():
             "frac": round(build_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
         },
         "stages": stages,
+        "sort_shape": list(tc.tune.shape(ctx, 32, False, n)),
+        "sort_tuning": tuning,
     }
     tree.free()
     if not args.no_extras:

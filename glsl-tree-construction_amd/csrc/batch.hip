@@ -534,8 +534,10 @@ extern "C" int tc_batch_create(tc_ctx* ctx, uint32_t cap_clouds, uint32_t cap_po
     b->cap_clouds = cap_clouds;
     b->cap_points = cap_points;
     b->code_bits = code_bits;
-    b->block = ctx->sort_block;
-    b->kpt = ctx->sort_kpt;
+    // Launch shape of the binning passes: the tuned one for the average cloud.
+    const SortShape sh = tc_sort_shape(ctx, code_bits == 30 ? 4 : 8, false, cap_points / cap_clouds);
+    b->block = sh.block;
+    b->kpt = sh.kpt;
     b->tile = (uint32_t)(b->block * b->kpt);
     b->cap_tiles = ceil_div_u32(cap_points, b->tile) + cap_clouds;
     const size_t C = cap_clouds, N = cap_points ? cap_points : 1;

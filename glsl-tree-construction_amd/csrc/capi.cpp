@@ -154,6 +154,7 @@ extern "C" int tc_create(tc_ctx** out, int device) {
     tc_ctx* ctx = new (std::nothrow) tc_ctx();
     if (!ctx) return TC_ERR_NOMEM;
     ctx->device = device;
+    std::memset(ctx->tune, -1, sizeof(ctx->tune));
     if (hipSetDevice(device) != hipSuccess) {
         delete ctx;
         return TC_ERR_HIP;
@@ -203,6 +204,7 @@ extern "C" int tc_set_sort_params(tc_ctx* ctx, int block, int kpt) {
     if (kpt != 8 && kpt != 12 && kpt != 16 && kpt != 20) return TC_ERR_INVALID;
     ctx->sort_block = block;
     ctx->sort_kpt = kpt;
+    ctx->tune_on = false;  // an explicit shape replaces tuned ones
     return TC_OK;
 }
 

@@ -23,6 +23,8 @@
 //    are not coherent with each other; one thread per digit walks back, four
 //    predecessors per round trip, with a bounded wait and a self-computed
 //    fall-back.  Each pass clears the other look-back region for the next pass.
+#include <cstring>
+
 #include "sort_core.hpp"
 
 namespace {
@@ -216,12 +218,12 @@ void launch_pass(uint32_t tiles, hipStream_t s, const K* sk, K* dk, const uint32
 }
 
 template <typename K, bool PAIRS>
-int dispatch_pass(tc_ctx* ctx, uint32_t tiles, hipStream_t s, const K* sk, K* dk, const uint32_t* sv,
-                  uint32_t* dv, uint32_t n, uint32_t shift, const uint32_t* prefix, uint32_t* lb_cur,
-                  uint32_t* lb_next, uint32_t* /*unused*/) {
+int dispatch_pass(tc_ctx* ctx, SortShape sh, uint32_t tiles, hipStream_t s, const K* sk, K* dk,
+                  const uint32_t* sv, uint32_t* dv, uint32_t n, uint32_t shift, const uint32_t* prefix,
+                  uint32_t* lb_cur, uint32_t* lb_next) {
     ProfScope prof(ctx, s, K_BINNING);
 #define TC_PASS_CASE(B, KP)                                                                         \
-    if (ctx->sort_block == B && ctx->sort_kpt == KP) {                                              \
+    if (sh.block == B && sh.kpt == KP) {                                                            \
         launch_pass<K, PAIRS, B, KP>(tiles, s, sk, dk, sv, dv, n, shift, prefix, lb_cur, lb_next,   \
                                      ctx->spin_limit);                                              \
         TC_LAUNCH_CHECK();                                                                           \
@@ -253,8 +255,8 @@ struct SortScratch {
     uint32_t* hist;
 };
 
-int sort_scratch(tc_ctx* ctx, uint32_t n, int digits, SortScratch* ss, uint32_t* tiles_out) {
-    const uint32_t tile = (uint32_t)(ctx->sort_block * ctx->sort_kpt);
+int sort_scratch(tc_ctx* ctx, uint32_t n, int digits, SortShape sh, SortScratch* ss, uint32_t* tiles_out) {
+    const uint32_t tile = (uint32_t)(sh.block * sh.kpt);
     const uint32_t tiles = ceil_div_u32(n, tile);
     const size_t region_words = (size_t)tiles * 256;
     const size_t words = 64 + 2 * region_words + (size_t)digits * 256;
@@ -290,9 +292,10 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
     if (n == 0) return TC_OK;
     if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
     constexpr int DIG = (int)sizeof(K);
+    const SortShape sh = tc_sort_shape(ctx, DIG, vals != nullptr, n);
     SortScratch ss;
     uint32_t tiles;
-    TC_TRY(sort_scratch(ctx, n, DIG, &ss, &tiles));
+    TC_TRY(sort_scratch(ctx, n, DIG, sh, &ss, &tiles));
     const uint32_t region_words = tiles * 256;
     // The histogram kernel also clears the tile counters and look-back region 0.
     if (pts) {
@@ -313,11 +316,11 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
         uint32_t* cur = ss.region[p & 1];
         uint32_t* nxt = (p + 1 < DIG) ? ss.region[(p + 1) & 1] : nullptr;
         if (vals) {
-            TC_TRY((dispatch_pass<K, true>(ctx, tiles, s, src, dst, vsrc, vdst, n, 8u * p, ss.hist + 256 * p,
-                                           cur, nxt, ss.ctr + p)));
+            TC_TRY((dispatch_pass<K, true>(ctx, sh, tiles, s, src, dst, vsrc, vdst, n, 8u * p, ss.hist + 256 * p,
+                                           cur, nxt)));
         } else {
-            TC_TRY((dispatch_pass<K, false>(ctx, tiles, s, src, dst, nullptr, nullptr, n, 8u * p,
-                                            ss.hist + 256 * p, cur, nxt, ss.ctr + p)));
+            TC_TRY((dispatch_pass<K, false>(ctx, sh, tiles, s, src, dst, nullptr, nullptr, n, 8u * p,
+                                            ss.hist + 256 * p, cur, nxt)));
         }
         K* t = src; src = dst; dst = t;
         uint32_t* tv = vsrc; vsrc = vdst; vdst = tv;
@@ -334,15 +337,16 @@ int binning_pass_impl(tc_ctx* ctx, const K* sk, K* dk, const uint32_t* sv, uint3
     if (shift % 8 != 0 || shift >= 8 * sizeof(K)) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
     if (n == 0) return TC_OK;
+    const SortShape sh = tc_sort_shape(ctx, (int)sizeof(K), sv != nullptr, n);
     SortScratch ss;
     uint32_t tiles;
-    TC_TRY(sort_scratch(ctx, n, (int)sizeof(K), &ss, &tiles));
+    TC_TRY(sort_scratch(ctx, n, (int)sizeof(K), sh, &ss, &tiles));
     TC_TRY(tc_fill_u32(ctx, ss.ctr, 0u, 64 + (size_t)tiles * 256, s));
     const uint32_t* prefix = hist + 256 * (shift / 8);
-    if (sv) return dispatch_pass<K, true>(ctx, tiles, s, sk, dk, sv, dv, n, shift, prefix, ss.region[0],
-                                          nullptr, ss.ctr);
-    return dispatch_pass<K, false>(ctx, tiles, s, sk, dk, nullptr, nullptr, n, shift, prefix, ss.region[0],
-                                   nullptr, ss.ctr);
+    if (sv) return dispatch_pass<K, true>(ctx, sh, tiles, s, sk, dk, sv, dv, n, shift, prefix, ss.region[0],
+                                          nullptr);
+    return dispatch_pass<K, false>(ctx, sh, tiles, s, sk, dk, nullptr, nullptr, n, shift, prefix, ss.region[0],
+                                   nullptr);
 }
 
 template <typename K>
@@ -402,4 +406,151 @@ extern "C" int tc_sort_pairs_u64(tc_ctx* ctx, uint64_t* keys, uint32_t* vals, ui
                                  uint32_t n, tc_stream stream) {
     if (!vals) return TC_ERR_INVALID;
     return sort_impl<uint64_t>(ctx, keys, vals, alt, valt, n, as_stream(stream));
+}
+
+// ------------------------------------------------------------------ autotuner
+namespace {
+
+__global__ void k_random_keys(uint32_t* __restrict__ out, size_t words, uint32_t seed) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t x = (uint32_t)i * 0x9E3779B9u ^ seed;
+        x ^= x >> 16;
+        x *= 0x7FEB352Du;
+        x ^= x >> 15;
+        x *= 0x846CA68Bu;
+        x ^= x >> 16;
+        out[i] = x;
+    }
+}
+
+int log2_bucket(uint32_t n) {
+    int b = 0;
+    while (b < 31 && (1u << b) < n) ++b;
+    return b;
+}
+
+}  // namespace
+
+SortShape tc_sort_shape(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n) {
+    if (ctx->tune_on) {
+        const int kb = key_bytes == 8 ? 1 : 0, pv = pairs ? 1 : 0;
+        const int b = log2_bucket(n);
+        for (int d = 0; d < 32; ++d) {
+            for (int x : {b - d, b + d}) {
+                if (x < 0 || x > 31) continue;
+                const int s = ctx->tune[kb][pv][x];
+                if (s >= 0 && s < N_SORT_SHAPES) return SORT_SHAPES[s];
+            }
+        }
+    }
+    return SortShape{ctx->sort_block, ctx->sort_kpt};
+}
+
+// Time every launch shape on random keys at sizes 2^12, 2^14, ... up to max_n,
+// for u32 and u64 keys, with and without a u32 payload, and keep the fastest
+// per size.  Blocking; uses its own stream and buffers.
+extern "C" int tc_autotune_sort(tc_ctx* ctx, uint32_t max_n) {
+    if (!ctx || max_n < (1u << 12)) return TC_ERR_INVALID;
+    if (max_n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    TC_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = nullptr;
+    TC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const size_t n = max_n;
+    void *k0 = nullptr, *k = nullptr, *ka = nullptr, *v = nullptr, *va = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = TC_OK;
+    if (hipMalloc(&k0, n * 8) != hipSuccess || hipMalloc(&k, n * 8) != hipSuccess ||
+        hipMalloc(&ka, n * 8) != hipSuccess || hipMalloc(&v, n * 4) != hipSuccess ||
+        hipMalloc(&va, n * 4) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        st = TC_ERR_NOMEM;
+    if (st == TC_OK) {
+        hipLaunchKernelGGL(k_random_keys, dim3(ctx->num_cus * 4), dim3(256), 0, s, static_cast<uint32_t*>(k0), n * 2,
+                           0x5EEDu);
+        if (hipGetLastError() != hipSuccess) st = TC_ERR_HIP;
+    }
+    const bool prof = ctx->prof_on;
+    ctx->prof_on = false;
+    const int saved_block = ctx->sort_block, saved_kpt = ctx->sort_kpt;
+    const bool saved_on = ctx->tune_on;
+    ctx->tune_on = false;
+    int8_t table[2][2][32];
+    std::memset(table, -1, sizeof(table));
+    for (int kb = 0; kb < 2 && st == TC_OK; ++kb) {
+        for (int pv = 0; pv < 2 && st == TC_OK; ++pv) {
+            for (int lg = 12; lg <= 30 && (1u << lg) <= max_n && st == TC_OK; lg += 2) {
+                const uint32_t m = 1u << lg;
+                float best = 1e30f;
+                int best_s = -1;
+                for (int sh = 0; sh < N_SORT_SHAPES && st == TC_OK; ++sh) {
+                    ctx->sort_block = SORT_SHAPES[sh].block;
+                    ctx->sort_kpt = SORT_SHAPES[sh].kpt;
+                    float t_min = 1e30f;
+                    for (int rep = 0; rep < 4 && st == TC_OK; ++rep) {
+                        const size_t kbytes = (size_t)m * (kb ? 8 : 4);
+                        if (hipMemcpyAsync(k, k0, kbytes, hipMemcpyDeviceToDevice, s) != hipSuccess) st = TC_ERR_HIP;
+                        if (st != TC_OK) break;
+                        (void)hipEventRecord(e0, s);
+                        if (kb == 0)
+                            st = pv ? tc_sort_pairs_u32(ctx, (uint32_t*)k, (uint32_t*)v, (uint32_t*)ka, (uint32_t*)va, m, s)
+                                    : tc_sort_keys_u32(ctx, (uint32_t*)k, (uint32_t*)ka, m, s);
+                        else
+                            st = pv ? tc_sort_pairs_u64(ctx, (uint64_t*)k, (uint32_t*)v, (uint64_t*)ka, (uint32_t*)va, m, s)
+                                    : tc_sort_keys_u64(ctx, (uint64_t*)k, (uint64_t*)ka, m, s);
+                        (void)hipEventRecord(e1, s);
+                        if (st == TC_OK && hipEventSynchronize(e1) != hipSuccess) st = TC_ERR_HIP;
+                        float ms = 0.f;
+                        if (st == TC_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = TC_ERR_HIP;
+                        if (rep > 0 && ms < t_min) t_min = ms;  // rep 0 warms up
+                    }
+                    if (t_min < best) {
+                        best = t_min;
+                        best_s = sh;
+                    }
+                }
+                table[kb][pv][lg] = (int8_t)best_s;
+            }
+        }
+    }
+    ctx->prof_on = prof;
+    ctx->sort_block = saved_block;
+    ctx->sort_kpt = saved_kpt;
+    if (st == TC_OK) {
+        std::memcpy(ctx->tune, table, sizeof(table));
+        ctx->tune_on = true;
+    } else {
+        ctx->tune_on = saved_on;
+    }
+    (void)hipStreamSynchronize(s);
+    for (void* p : {k0, k, ka, v, va})
+        if (p) (void)hipFree(p);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    return st;
+}
+
+extern "C" int tc_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, uint32_t n, int* block, int* kpt) {
+    if (!ctx || !block || !kpt || (key_bits != 32 && key_bits != 64)) return TC_ERR_INVALID;
+    const SortShape sh = tc_sort_shape(ctx, key_bits / 8, pairs != 0, n);
+    *block = sh.block;
+    *kpt = sh.kpt;
+    return TC_OK;
+}
+
+extern "C" int tc_set_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, int log2_n, int block, int kpt) {
+    if (!ctx || (key_bits != 32 && key_bits != 64) || log2_n < 0 || log2_n > 31) return TC_ERR_INVALID;
+    int idx = -1;
+    for (int i = 0; i < N_SORT_SHAPES; ++i)
+        if (SORT_SHAPES[i].block == block && SORT_SHAPES[i].kpt == kpt) idx = i;
+    if (idx < 0) return TC_ERR_INVALID;
+    ctx->tune[key_bits == 64 ? 1 : 0][pairs ? 1 : 0][log2_n] = (int8_t)idx;
+    ctx->tune_on = true;
+    return TC_OK;
+}
+
+extern "C" int tc_clear_sort_tuning(tc_ctx* ctx) {
+    if (!ctx) return TC_ERR_INVALID;
+    std::memset(ctx->tune, -1, sizeof(ctx->tune));
+    ctx->tune_on = false;
+    return TC_OK;
 }

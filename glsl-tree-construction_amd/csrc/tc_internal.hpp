@@ -69,9 +69,13 @@ struct tc_ctx {
     double prof_ms[K_COUNT] = {};
     uint64_t prof_n[K_COUNT] = {};
     int num_cus = 256;
-    // Sort launch parameters.
+    // Sort launch parameters (used when no tuned shape applies).
     int sort_block = 512;
     int sort_kpt = 20;
+    // Tuned launch shapes (tc_autotune_sort): index into SORT_SHAPES per
+    // [key bytes 4 / 8][keys only / pairs][ceil(log2 n)], -1 where not measured.
+    bool tune_on = false;
+    int8_t tune[2][2][32];
     // Look-back wait bound (empty polls) before a workgroup computes its
     // prefix itself; tests set it to 0 to exercise that path.
     uint32_t spin_limit = 1u << 15;
@@ -98,6 +102,18 @@ constexpr size_t SMALL_WORDS = 4096;
 // Scratch sizing helpers (bytes).
 int tc_ensure_scratch(tc_ctx* ctx, size_t bytes);
 size_t tc_sort_scratch_bytes(uint32_t n, int passes, int tile);
+
+// Binning-pass launch shapes (workgroup size, keys per lane) the library is
+// built with; the tuner picks among them.
+struct SortShape {
+    int block, kpt;
+};
+constexpr int N_SORT_SHAPES = 8;
+constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {{256, 8}, {256, 12}, {256, 16}, {256, 20},
+                                                  {512, 8}, {512, 12}, {512, 16}, {512, 20}};
+// The shape a sort of n keys uses: the tuned one for the nearest measured size
+// when the context has been tuned, else the context's fixed shape.
+SortShape tc_sort_shape(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n);
 size_t tc_scan_scratch_bytes(uint32_t n);
 
 static inline hipStream_t as_stream(tc_stream s) { return reinterpret_cast<hipStream_t>(s); }

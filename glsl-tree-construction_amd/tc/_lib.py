@@ -186,6 +186,10 @@ SIGNATURES = {
     "tc_profile_get": (I, [P, I, C.POINTER(C.c_double), C.POINTER(U64)]),
     "tc_kernel_name": (C.c_char_p, [I]),
     "tc_get_sort_params": (I, [P, C.POINTER(I), C.POINTER(I)]),
+    "tc_autotune_sort": (I, [P, U32]),
+    "tc_tuned_sort_shape": (I, [P, I, I, U32, C.POINTER(I), C.POINTER(I)]),
+    "tc_set_tuned_sort_shape": (I, [P, I, I, I, I, I]),
+    "tc_clear_sort_tuning": (I, [P]),
     "tc_morton30": (I, [P, P, P, U32, F, F, S]),
     "tc_morton63": (I, [P, P, P, U32, F, F, S]),
     "tc_morton30_decode": (I, [P, P, P, U32, F, F, S]),

@@ -1,0 +1,55 @@
+"""Autotuner of the sort's launch shape (tc_autotune_sort).
+
+The binning pass is built for eight (workgroup size, keys per lane) shapes;
+which is fastest depends on the key width, the payload and the size.
+``autotune`` times them all on the device and the context then picks, for each
+sort, the shape measured at the nearest size (batches: at their average cloud
+size).  The table can be saved as JSON and restored into another context.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+
+from ._lib import check, lib
+
+VARIANTS = {"u32_keys": (32, 0), "u32_pairs": (32, 1), "u64_keys": (64, 0), "u64_pairs": (64, 1)}
+
+
+def autotune(ctx, max_n: int = 1 << 24) -> dict:
+    check(lib().tc_autotune_sort(ctx.handle, max_n), "tc_autotune_sort")
+    return table(ctx, max_n)
+
+
+def shape(ctx, key_bits: int, pairs: bool, n: int) -> tuple:
+    b, k = C.c_int(), C.c_int()
+    check(lib().tc_tuned_sort_shape(ctx.handle, key_bits, int(bool(pairs)), n, C.byref(b), C.byref(k)),
+          "tc_tuned_sort_shape")
+    return b.value, k.value
+
+
+def table(ctx, max_n: int = 1 << 30) -> dict:
+    """Shape per variant and size bucket (log2 n, even buckets up to max_n)."""
+    out = {}
+    for name, (bits, pairs) in VARIANTS.items():
+        out[name] = {lg: list(shape(ctx, bits, pairs, 1 << lg)) for lg in range(12, 31, 2) if (1 << lg) <= max_n}
+    return out
+
+
+def save(ctx, path: str, max_n: int = 1 << 30):
+    with open(path, "w") as f:
+        json.dump({k: {str(lg): v for lg, v in t.items()} for k, t in table(ctx, max_n).items()}, f, indent=1)
+
+
+def load(ctx, path: str):
+    with open(path) as f:
+        t = json.load(f)
+    for name, entries in t.items():
+        bits, pairs = VARIANTS[name]
+        for lg, (block, kpt) in entries.items():
+            check(lib().tc_set_tuned_sort_shape(ctx.handle, bits, pairs, int(lg), block, kpt),
+                  "tc_set_tuned_sort_shape")
+
+
+def clear(ctx):
+    check(lib().tc_clear_sort_tuning(ctx.handle), "tc_clear_sort_tuning")

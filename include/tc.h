@@ -77,6 +77,17 @@ int tc_reserve(tc_ctx* ctx, uint32_t max_n);
  * keys per thread (8, 12, 16 or 20).  0 keeps the current value. */
 int tc_set_sort_params(tc_ctx* ctx, int block, int keys_per_thread);
 int tc_get_sort_params(tc_ctx* ctx, int* block, int* keys_per_thread);
+/* Autotuner (no reference counterpart): times every binning-pass launch shape
+ * on random keys at sizes 2^12, 2^14, ... up to max_n, for u32 and u64 keys with
+ * and without a u32 payload, and from then on each sort uses the fastest shape
+ * measured at the size nearest its own (batches: at their average cloud size).
+ * Blocking.  tc_set_sort_params switches back to one fixed shape. */
+int tc_autotune_sort(tc_ctx* ctx, uint32_t max_n);
+/* The shape a sort of n keys (key_bits 32 or 64, pairs 0/1) would use now. */
+int tc_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, uint32_t n, int* block, int* keys_per_thread);
+/* Restore one entry of a saved tuning table (size bucket ceil(log2 n)). */
+int tc_set_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, int log2_n, int block, int keys_per_thread);
+int tc_clear_sort_tuning(tc_ctx* ctx);
 /* Bound on a workgroup's look-back wait (empty polls of a predecessor's
  * status) in the sort and scan kernels; past it the workgroup computes its
  * prefix from the input itself, so progress never depends on workgroup start

@@ -1,0 +1,45 @@
+"""Autotuner: it measures, the table it leaves is made of built shapes, and every
+sort it steers (any size, keys or pairs, u32 or u64) still matches the oracle."""
+import numpy as np
+import pytest
+
+from gpu_util import dev, empty, host
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = {(b, k) for b in (256, 512) for k in (8, 12, 16, 20)}
+
+
+def test_autotune_table_and_sorts(oracle, tclib, tmp_path):
+    import tc
+
+    ctx = tc.Context(0)
+    try:
+        t = tc.tune.autotune(ctx, 1 << 20)
+        assert set(t) == {"u32_keys", "u32_pairs", "u64_keys", "u64_pairs"}
+        for entries in t.values():
+            assert entries and all(tuple(v) in SHAPES for v in entries.values())
+        r = np.random.default_rng(21)
+        for n in (1000, 70_000, 1 << 20):
+            keys = r.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+            vals = np.arange(n, dtype=np.uint32)
+            k, v = dev(keys), dev(vals)
+            ctx.sort(k, empty(n, np.uint32), v, empty(n, np.uint32))
+            sk, sv = oracle.radix_sort(keys, vals)
+            assert np.array_equal(host(k, np.uint32), sk) and np.array_equal(host(v, np.uint32), sv)
+            k64 = r.integers(0, 1 << 63, size=n, dtype=np.uint64)
+            k = dev(k64)
+            ctx.sort(k, empty(n, np.uint64))
+            assert np.array_equal(host(k, np.uint64), np.sort(k64))
+        # Save / clear / load round trip.
+        path = str(tmp_path / "tune.json")
+        tc.tune.save(ctx, path, 1 << 20)
+        tc.tune.clear(ctx)
+        assert tc.tune.shape(ctx, 32, False, 1 << 20) == ctx.sort_params()
+        tc.tune.load(ctx, path)
+        assert tc.tune.table(ctx, 1 << 20) == t
+        # An explicit shape replaces the tuned ones.
+        ctx.set_sort_params(256, 12)
+        assert tc.tune.shape(ctx, 64, True, 1 << 16) == (256, 12)
+    finally:
+        ctx.close()
