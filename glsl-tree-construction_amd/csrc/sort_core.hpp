@@ -41,10 +41,14 @@ template <typename K, int BLOCK, int KPT>
 struct OnesweepShared {
     static constexpr int WAVES = BLOCK / 64;
     static constexpr uint32_t TILE = BLOCK * KPT;
-    uint32_t whist[WAVES * 256];
+    // Per-wave digit counters, then tile-local positions (< TILE <= 65535,
+    // so 16 bits); the fall-back digit counts reuse the space afterwards.
+    union {
+        uint16_t whist[WAVES * 256];
+        uint32_t fb[256];
+    };
     uint32_t gbase[256];
     uint32_t tmp[WAVES + 2];
-    uint32_t fb[256];  // fall-back digit counts
     uint32_t tile;
     uint32_t aux;
     uint32_t fallback;
@@ -78,6 +82,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
     constexpr bool full = FULL;  // full tiles (cnt == TILE) skip every bounds test
     const uint32_t wofs = wave * WAVE_TILE + lane;  // tile-relative index of key 0 of this lane
 
+    static_assert(BLOCK * KPT <= 65535, "tile positions must fit the 16-bit counters");
     for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) sm.whist[i] = 0u;
 
     K keys[KPT];
@@ -108,7 +113,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
     // observes the leader updates of iteration i.  The signal fence keeps the
     // compiler from moving the LDS accesses across iterations (it emits no
     // instruction; a volatile pointer here would turn them into flat accesses).
-    uint32_t* wh = sm.whist + wave * 256;
+    uint16_t* wh = sm.whist + wave * 256;
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
         const uint32_t d = digit_of<K>(keys[i], shift);
@@ -117,7 +122,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
         const uint32_t lt = mbcnt64(peers);
         const uint32_t before = wh[d];
         pos[i] = before + lt;
-        if (valid && lt == 0) wh[d] = before + (uint32_t)__popcll(peers);
+        if (valid && lt == 0) wh[d] = (uint16_t)(before + (uint32_t)__popcll(peers));
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
     }
     __syncthreads();
@@ -129,7 +134,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) {
             const uint32_t c = sm.whist[w * 256 + tid];
-            sm.whist[w * 256 + tid] = run;
+            sm.whist[w * 256 + tid] = (uint16_t)run;
             run += c;
         }
         total_d = run;
@@ -140,7 +145,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
     if (tid < 256) {
         // whist[w][d] becomes the tile-local position of wave w's first digit-d key.
 #pragma unroll
-        for (int w = 0; w < WAVES; ++w) sm.whist[w * 256 + tid] += start;
+        for (int w = 0; w < WAVES; ++w) sm.whist[w * 256 + tid] = (uint16_t)(sm.whist[w * 256 + tid] + start);
     }
     __syncthreads();
 
