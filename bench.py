@@ -34,6 +34,43 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "glsl-tree-construction_amd"))
 
 HBM_PEAK_GBS = 8000.0
+TUNING_FILE = os.path.join(ROOT, "glsl-tree-construction_amd", "tc", "tuning_mi355x.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "binning_pass_2p26_gauss_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per binning-pass launch at the headline workload, from the
+    committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (None if absent)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def traffic_probe(ctx, pts_h):
+    """Three headline builds, then reference kernels of known traffic on a
+    256 MiB int32 tensor: fill_ writes 256 MiB, sum reads 256 MiB."""
+    import torch
+
+    import tc
+
+    n = len(pts_h)
+    pts = torch.from_numpy(pts_h).cuda()
+    tree = tc.Tree(ctx, n, 30)
+    for _ in range(3):
+        tree.build(pts, n, 0.0, 1024.0)
+    a = torch.empty(1 << 26, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        a.fill_(1)
+    total = 0
+    for _ in range(3):
+        total += int(a.sum())
+    torch.cuda.synchronize()
+    print(json.dumps({"traffic_probe": tree.read_counts(), "check": total}), flush=True)
+    tree.free()
 METRIC = "Mpoints/s full tree build @2^26 (% HBM roofline); batched vs plain p99 latency"
 
 
@@ -225,8 +262,8 @@ def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04):
     }
 
 
-def bench_served(ctx, rates=(1000, 2000, 5000, 10000, 20000, 50000), pool=256, max_req=2000, window_s=0.5,
-                 slo_ms=10.0, seed=0xC0FFEE05):
+def bench_served(ctx, rates=(1000, 2000, 5000, 10000, 12000, 14000, 16000, 20000, 50000), pool=256, max_req=2000,
+                 window_s=0.5, slo_ms=10.0, seed=0xC0FFEE05):
     """Poisson arrivals of build requests (sizes log-uniform in [2^12, 2^22],
     uniform xyz in [0,1024)^3) against the server (64 GiB budget, 2 streams) and
     against a one-request-at-a-time plain loop, on the same traces.  Inputs are
@@ -294,7 +331,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline build only")
     ap.add_argument("--profile-steps", type=int, default=3)
-    ap.add_argument("--no-tune", action="store_true", help="fixed sort launch shape instead of the autotuner's")
+    ap.add_argument("--no-tune", action="store_true", help="fixed sort launch shape instead of the tuned ones")
+    ap.add_argument("--retune", action="store_true", help="run the autotuner instead of loading its saved table")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only build the headline tree 3 times, then run reference kernels of known traffic "
+                         "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -308,7 +349,15 @@ def main():
     n = 1 << args.log2n
     pts_h = tc.gen_gaussian_mixture(n, args.seed, 64, 8.0, 1024.0)
     ctx = tc.Context(0)
-    tuning = None if args.no_tune else tc.tune.autotune(ctx, n)
+    tuning = None
+    if args.retune:
+        tuning = tc.tune.autotune(ctx, n)
+    elif not args.no_tune and os.path.exists(TUNING_FILE):
+        tc.tune.load(ctx, TUNING_FILE)
+        tuning = "loaded " + os.path.relpath(TUNING_FILE, ROOT)
+    if args.traffic_probe:
+        traffic_probe(ctx, pts_h)
+        return
     pts = torch.from_numpy(pts_h).cuda()
     tree = tc.Tree(ctx, n, 30)
 
@@ -360,7 +409,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": d["roofline_frac"],
-            "traffic": None,
+            "traffic": measured_traffic() if (args.log2n == 26 and dom == "binning") else None,
             "alg_bytes_per_launch": d["alg_bytes"],
             "avg_launch_ms": d["avg_ms"],
         },
