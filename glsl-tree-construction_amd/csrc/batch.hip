@@ -75,7 +75,7 @@ constexpr uint32_t SEG_BRT_LIMIT = 64;  // radix ranges searched in LDS only (wi
 
 // Plan views.
 constexpr uint32_t PLAN_IN_PTRS = 1u;   // desc[3]: cloud c is read from in_ptr[c]
-constexpr uint32_t PLAN_OUT_PTRS = 2u;  // desc[3]: outputs are copied to out[c]
+constexpr uint32_t PLAN_OUT_PTRS = 2u;  // desc[3]: outputs are written to out[c] (nodes only there)
 
 struct Plan {
     const uint32_t* desc;  // [0] n_clouds [1] n_tiles [2] n_points [3] flags
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_finalize(const uint32_t* __re
 // Unique-key slots: radix tree (and edge counts) per cloud.  The last slot of
 // each cloud has no radix node; its edge count is 0 so the node scan skips it.
 template <typename K>
-__global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(const uint32_t* __restrict__ desc, const uint32_t* __restrict__ ubase,
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* __restrict__ ubase,
                                                        const K* __restrict__ uniq, uint8_t* __restrict__ prefix_n,
                                                        uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr,
                                                        int32_t* __restrict__ left, int32_t* __restrict__ parent,
@@ -277,14 +277,22 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(const uint32_t* __restric
     // Tiles of SEG_BRT_TILE slots with the unique codes of the tile and a halo
     // staged in LDS (a cache of the concatenated array: each cloud still reads
     // only its own codes, by cloud-local index).  Bounded searches first, the
-    // long ranges afterwards (see k_build_radix_tree).
+    // long ranges afterwards (see k_build_radix_tree).  With per-cloud output
+    // buffers (tc_batch_set_io) the unique codes and radix arrays are also
+    // stored there.
     __shared__ K s_win[SEG_BRT_TILE + 2 * SEG_SEARCH_HALO];
     __shared__ uint32_t s_long[SEG_BRT_TILE];
     __shared__ uint32_t s_nlong;
     __shared__ uint32_t s_c0;
-    const uint32_t C = desc[0];
+    const uint32_t C = p.desc[0];
+    const bool to_clouds = (p.desc[3] & PLAN_OUT_PTRS) != 0;
     const uint32_t total = ubase[C];
     const uint32_t tiles = ceil_div_u32(total, SEG_BRT_TILE);
+    auto arrays = [&](uint32_t b) { return BrtArrays{prefix_n + b, hll + b, hlr + b, left + b, parent + b, ec + b}; };
+    auto cloud_arrays = [&](uint32_t c) {
+        const tc_tree_ptrs& o = p.out[c];
+        return BrtArrays{o.prefix_n, o.has_leaf_left, o.has_leaf_right, o.left_child, o.parent, o.edge_count};
+    };
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const int64_t t0 = (int64_t)t * SEG_BRT_TILE;
         const int64_t w0 = t0 > SEG_SEARCH_HALO ? t0 - SEG_SEARCH_HALO : 0;
@@ -304,11 +312,16 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(const uint32_t* __restric
             const uint32_t c = seg_find_near(ubase, C, g, c0);
             const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
             const uint32_t i = g - b;
+            if (to_clouds) static_cast<K*>(p.out[c].unique)[i] = s_win[g - w0];
             if (i + 1 < nk) {
-                if (!brt_node_near<K, SEG_BRT_LIMIT, true>(as_lds<K>(s_win), (int32_t)(w0 - (int64_t)b), nk, i,
-                                                           prefix_n + b, hll + b, hlr + b, left + b, parent + b,
-                                                           ec + b))
+                BrtResult r;
+                if (brt_node_near<K, SEG_BRT_LIMIT, true>(as_lds<K>(s_win), (int32_t)(w0 - (int64_t)b), nk, i, true,
+                                                          &r)) {
+                    brt_store(arrays(b), i, r);
+                    if (to_clouds) brt_store(cloud_arrays(c), i, r);
+                } else {
                     s_long[atomicAdd(&s_nlong, 1u)] = g;
+                }
             } else {
                 ec[g] = 0u;
             }
@@ -320,7 +333,10 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(const uint32_t* __restric
             const uint32_t c = seg_find_near(ubase, C, g, c0);
             const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
             const CodeView<K> view{uniq + b, as_lds<K>(s_win), (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0)};
-            brt_node<K>(view, nk, g - b, prefix_n + b, hll + b, hlr + b, left + b, parent + b, ec + b);
+            BrtResult r;
+            brt_node<K>(view, nk, g - b, true, &r);
+            brt_store(arrays(b), g - b, r);
+            if (to_clouds) brt_store(cloud_arrays(c), g - b, r);
         }
     }
 }
@@ -355,6 +371,16 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
         }
         __syncthreads();
         const uint32_t c0 = s_c0;
+        const bool to_clouds = (p.desc[3] & PLAN_OUT_PTRS) != 0;
+        if (to_clouds) {  // cloud-local node offsets into the cloud's own buffer
+#pragma unroll 1
+            for (int k = 0; k < SEG_BRT_TILE / SEG_BLOCK; ++k) {
+                const uint32_t g = (uint32_t)t0 + k * SEG_BLOCK + threadIdx.x;
+                if (g >= total) break;
+                const uint32_t c = seg_find_near(ubase, C, g, c0);
+                p.out[c].offsets[g - ubase[c]] = offsets[g] - node_base[c];
+            }
+        }
         // Compact the slots with ec > 0 (the last slot of a cloud has ec 0).
         uint32_t count = 0;
 #pragma unroll 1
@@ -376,46 +402,13 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
             if (i + 1 >= nk) continue;
             const LinkView lv{left + b, hll + b, hlr + b, ec + b, as_lds<int32_t>(s_left), as_lds<uint16_t>(s_meta),
                               (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0)};
-            octree_build_node<K>(nodes, node_cap, node_base[c], offsets + b, ec + b, uniq + b, prefix_n + b, lv, i,
-                                 nk - 1, prefix_n[b] / 3, link_leaves, p.min_c[c], p.range_c[c]);
+            // Nodes go to the cloud's own buffer when there is one (then only there).
+            tc_oct_node* dst = to_clouds ? p.out[c].nodes : nodes;
+            const uint32_t cap = to_clouds ? d_octree_node_bound(nk, CodeTraits<K>::BITS) : node_cap;
+            octree_build_node<K>(dst, cap, node_base[c], offsets + b, ec + b, uniq + b, prefix_n + b, lv, i, nk - 1,
+                                 prefix_n[b] / 3, link_leaves, p.min_c[c], p.range_c[c],
+                                 to_clouds ? node_base[c] : 0u);
         }
-    }
-}
-
-// Copy every cloud's outputs into its own buffers (tc_batch_set_io), in the
-// layout of a plain build: offsets rebased to the cloud's first node.  Exits at
-// once when the plan has no output buffers, so the launch can sit in a
-// captured graph unconditionally.
-template <typename K>
-__global__ __launch_bounds__(SEG_BLOCK) void k_seg_copyout(
-    Plan p, const uint32_t* __restrict__ ubase, const uint32_t* __restrict__ node_base, const K* __restrict__ uniq,
-    const uint8_t* __restrict__ prefix_n, const uint8_t* __restrict__ hll, const uint8_t* __restrict__ hlr,
-    const int32_t* __restrict__ left, const int32_t* __restrict__ parent, const uint32_t* __restrict__ ec,
-    const uint32_t* __restrict__ offsets, const tc_oct_node* __restrict__ nodes) {
-    if (!(p.desc[3] & PLAN_OUT_PTRS)) return;
-    const uint32_t C = p.desc[0];
-    const uint32_t stride = gridDim.x * SEG_BLOCK;
-    const uint32_t total_u = ubase[C];
-    for (uint32_t g = blockIdx.x * SEG_BLOCK + threadIdx.x; g < total_u; g += stride) {
-        const uint32_t c = seg_find(ubase, C, g);
-        const uint32_t b = ubase[c], nk = ubase[c + 1] - b, i = g - b;
-        const tc_tree_ptrs o = p.out[c];
-        static_cast<K*>(o.unique)[i] = uniq[g];
-        o.offsets[i] = offsets[g] - node_base[c];
-        if (i + 1 < nk) {
-            o.prefix_n[i] = prefix_n[g];
-            o.has_leaf_left[i] = hll[g];
-            o.has_leaf_right[i] = hlr[g];
-            o.left_child[i] = left[g];
-            o.parent[i] = parent[g];
-            o.edge_count[i] = ec[g];
-        }
-    }
-    const uint64_t total_q = 4ull * node_base[C];  // 16-byte quarters of the nodes
-    const uint4* src = reinterpret_cast<const uint4*>(nodes);
-    for (uint64_t g = (uint64_t)blockIdx.x * SEG_BLOCK + threadIdx.x; g < total_q; g += stride) {
-        const uint32_t c = seg_find(node_base, C, (uint32_t)(g >> 2));
-        reinterpret_cast<uint4*>(p.out[c].nodes)[g - 4ull * node_base[c]] = src[g];
     }
 }
 
@@ -485,7 +478,7 @@ int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s
     hipLaunchKernelGGL(k_seg_finalize, dim3(fgrid), dim3(SEG_BLOCK), 0, s, p.desc, p.pt_off, b->ubase, b->totals);
     TC_LAUNCH_CHECK();
     const uint32_t tgrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK);
-    hipLaunchKernelGGL(k_seg_brt<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, uniq, b->prefix_n,
+    hipLaunchKernelGGL(k_seg_brt<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p, b->ubase, uniq, b->prefix_n,
                        b->hll, b->hlr, b->left, b->parent, b->ec);
     TC_LAUNCH_CHECK();
     TC_TRY(tc_launch_scan(ctx, MODE_SEG_SCAN, b->ec, nullptr, b->offsets, nullptr, b->totals + 1, b->cap_points,
@@ -496,9 +489,6 @@ int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s
     const uint32_t ncap = (uint32_t)(b->node_cap < 0xFFFFFFFFull ? b->node_cap : 0xFFFFFFFFull);
     hipLaunchKernelGGL(k_seg_octree<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, b->node_base,
                        b->nodes, ncap, b->offsets, b->ec, uniq, b->prefix_n, b->left, b->hll, b->hlr, link_leaves, p);
-    TC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_seg_copyout<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p, b->ubase, b->node_base, uniq,
-                       b->prefix_n, b->hll, b->hlr, b->left, b->parent, b->ec, b->offsets, b->nodes);
     TC_LAUNCH_CHECK();
     return TC_OK;
 }

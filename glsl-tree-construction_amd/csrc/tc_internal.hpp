@@ -173,6 +173,18 @@ int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block);
 
 TC_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
+// tc_octree_node_bound on the device: 1 + sum over levels of min(8^L, n/2).
+TC_DEV uint32_t d_octree_node_bound(uint32_t n_keys, int code_bits) {
+    if (n_keys < 2) return 0;
+    const uint64_t half = n_keys / 2;
+    uint64_t total = 1, p = 1;
+    for (int L = 1; L <= (code_bits - 1) / 3; ++L) {
+        p = p < (1ull << 60) ? p * 8 : p;
+        total += p < half ? p : half;
+    }
+    return total < 0xFFFFFFFFull ? (uint32_t)total : 0xFFFFFFFFu;
+}
+
 // popcount(mask & lanes-below-me), the wave64 exclusive bit count.
 TC_DEV uint32_t mbcnt64(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));

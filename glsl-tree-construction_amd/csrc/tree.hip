@@ -49,6 +49,7 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
     const uint32_t nb = resolve_nodes(n_keys_cap > 0 ? n_keys_cap - 1 : 0, d_n_keys);
     const int64_t nk = (int64_t)nb + 1;
     const uint32_t tiles = ceil_div_u32(nb, BRT_TILE);
+    const BrtArrays out{prefix_n, hll, hlr, left_child, parent, ec};
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         // Window [ws, ws + BRT_TILE + 2 BRT_HALO); slots outside [0, nk) hold
         // the all-ones sentinel ("no common prefix").
@@ -64,9 +65,12 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
 #pragma unroll 1
         for (int k = 0; k < BRT_TILE / TREE_BLOCK; ++k) {
             const int64_t i = t0 + k * TREE_BLOCK + threadIdx.x;
-            if (i < nb && !brt_node_near<K, BRT_LIMIT, false>(as_lds<K>(s_win), (int32_t)ws, (uint32_t)nk,
-                                                              (uint32_t)i, prefix_n, hll, hlr, left_child, parent,
-                                                              ec))
+            if (i >= nb) continue;
+            BrtResult r;
+            if (brt_node_near<K, BRT_LIMIT, false>(as_lds<K>(s_win), (int32_t)ws, (uint32_t)nk, (uint32_t)i,
+                                                   ec != nullptr, &r))
+                brt_store(out, (uint32_t)i, r);
+            else
                 s_long[atomicAdd(&s_nlong, 1u)] = (uint32_t)i;
         }
         __syncthreads();
@@ -74,8 +78,11 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
         const int64_t w1 = ws + BRT_TILE + 2 * BRT_HALO < nk ? ws + BRT_TILE + 2 * BRT_HALO : nk;
         const CodeView<K> view{codes, as_lds<K>(s_win + (w0 - ws)), (int32_t)w0, (uint32_t)(w1 - w0)};
         const uint32_t nlong = s_nlong;
-        for (uint32_t q = threadIdx.x; q < nlong; q += TREE_BLOCK)
-            brt_node<K>(view, (uint32_t)nk, s_long[q], prefix_n, hll, hlr, left_child, parent, ec);
+        for (uint32_t q = threadIdx.x; q < nlong; q += TREE_BLOCK) {
+            BrtResult r;
+            brt_node<K>(view, (uint32_t)nk, s_long[q], ec != nullptr, &r);
+            brt_store(out, s_long[q], r);
+        }
     }
 }
 

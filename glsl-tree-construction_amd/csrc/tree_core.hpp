@@ -82,24 +82,57 @@ TC_DEV int log2_ceil_u32(uint32_t x) {
     return fl + (((uint32_t)1 << fl) < x ? 1 : 0);
 }
 
+// Outputs of one radix node: its prefix length, split, leaf flags and the edge
+// counts of its internal children.
+struct BrtResult {
+    int dnode;
+    uint32_t gamma;
+    bool leaf_l, leaf_r;
+    uint32_t ec_l, ec_r;
+};
+
+// The radix-tree arrays of one cloud (cloud-local indices); ec may be NULL.
+struct BrtArrays {
+    uint8_t* prefix_n;
+    uint8_t* hll;
+    uint8_t* hlr;
+    int32_t* left_child;
+    int32_t* parent;
+    uint32_t* ec;
+};
+
+// Store node i: its own entries, the parent links of its internal children,
+// and (with ec) their edge counts; node 0 also writes the root's parent slot
+// (the reference's zero-initialised entry, src/main.rs:103) and count, 1.
+TC_DEV void brt_store(const BrtArrays& o, uint32_t i, const BrtResult& r) {
+    o.prefix_n[i] = (uint8_t)r.dnode;
+    o.left_child[i] = (int32_t)r.gamma;
+    o.hll[i] = r.leaf_l;
+    o.hlr[i] = r.leaf_r;
+    if (!r.leaf_l) o.parent[r.gamma] = (int32_t)i;
+    if (!r.leaf_r) o.parent[r.gamma + 1] = (int32_t)i;
+    if (i == 0) o.parent[0] = 0;
+    if (o.ec) {
+        if (!r.leaf_l) o.ec[r.gamma] = r.ec_l;
+        if (!r.leaf_r) o.ec[r.gamma + 1] = r.ec_r;
+        if (i == 0) o.ec[0] = 1u;
+    }
+}
+
 // k_BuildRadixTree, shader/build_radix_tree.comp:71-146 (Karras 2012), for
 // internal node i of a cloud of nk strictly increasing codes.
 //
-// When ec is non-NULL the node also writes the edge count of its internal
-// children (countEdges, shader/edge_count.comp:36-38): a child covering
-// [lo, hi] has prefix length lcp(codes[lo], codes[hi]), so its count
-// prefix_n[child]/3 - prefix_n[i]/3 is known here without reading the child's
-// output; node 0 writes the root's count, 1.
+// With want_ec the node also computes the edge counts of its internal children
+// (countEdges, shader/edge_count.comp:36-38): a child covering [lo, hi] has
+// prefix length lcp(codes[lo], codes[hi]), so its count
+// prefix_n[child]/3 - prefix_n[i]/3 is known here without the child's output.
 //
-// With LIMIT > 0 the node gives up (returns false, writes nothing) when its
-// range reaches LIMIT keys: every probe then stays within LIMIT of i, so a
-// caller can serve them from an LDS window and handle the few long ranges
-// (nodes near the root) separately, instead of letting one long search hold
-// up a whole wave.
+// With LIMIT > 0 the node gives up (returns false) when its range reaches
+// LIMIT keys: every probe then stays within LIMIT of i, so a caller can serve
+// them from an LDS window and handle the few long ranges (nodes near the root)
+// separately, instead of letting one long search hold up a whole wave.
 template <typename K, uint32_t LIMIT = 0, typename View = CodeView<K>>
-TC_DEV bool brt_node(View codes, uint32_t nk, uint32_t i, uint8_t* __restrict__ prefix_n,
-                     uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr, int32_t* __restrict__ left_child,
-                     int32_t* __restrict__ parent, uint32_t* __restrict__ ec = nullptr) {
+TC_DEV bool brt_node(View codes, uint32_t nk, uint32_t i, bool want_ec, BrtResult* out) {
     const Delta<K, View> delta{codes, nk, codes[i]};
     // Direction (Karras 2012 eq. 1); ties (duplicate codes only) take +1.
     const bool fwd = delta(i + 1) - delta(i - 1) >= 0;
@@ -116,7 +149,6 @@ TC_DEV bool brt_node(View codes, uint32_t nk, uint32_t i, uint8_t* __restrict__ 
         if (delta(at(l + t)) > dmin) l += t;
     const uint32_t j = at(l);
     const int dnode = delta(j);
-    prefix_n[i] = (uint8_t)dnode;
     // Split: binary search with ceil(l / 2^k) steps (shader/build_radix_tree.comp:119-129).
     uint32_t s = 0;
     const int kmax = l <= 1 ? 0 : log2_ceil_u32(l);
@@ -124,28 +156,25 @@ TC_DEV bool brt_node(View codes, uint32_t nk, uint32_t i, uint8_t* __restrict__ 
         const uint32_t t = (l + (1u << k) - 1) >> k;
         if (delta(at(s + t)) > dnode) s += t;
     }
-    const uint32_t gamma = fwd ? i + s : i - s - 1;
-    left_child[i] = (int32_t)gamma;
+    BrtResult r;
+    r.dnode = dnode;
+    r.gamma = fwd ? i + s : i - s - 1;
     const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
-    const bool leaf_l = lo == gamma;
-    const bool leaf_r = hi == gamma + 1;
-    hll[i] = leaf_l;
-    hlr[i] = leaf_r;
-    if (!leaf_l) parent[gamma] = (int32_t)i;
-    if (!leaf_r) parent[gamma + 1] = (int32_t)i;
-    if (i == 0) parent[0] = 0;  // the reference's zero-initialised slot (src/main.rs:103)
-    if (ec) {
+    r.leaf_l = lo == r.gamma;
+    r.leaf_r = hi == r.gamma + 1;
+    r.ec_l = r.ec_r = 0;
+    if (want_ec) {
         const int level = dnode / 3;
-        if (!leaf_l) {
+        if (!r.leaf_l) {
             const Delta<K, View> dl{codes, nk, codes[lo]};
-            ec[gamma] = (uint32_t)(dl(gamma) / 3 - level);
+            r.ec_l = (uint32_t)(dl(r.gamma) / 3 - level);
         }
-        if (!leaf_r) {
-            const Delta<K, View> dr{codes, nk, codes[gamma + 1]};
-            ec[gamma + 1] = (uint32_t)(dr(hi) / 3 - level);
+        if (!r.leaf_r) {
+            const Delta<K, View> dr{codes, nk, codes[r.gamma + 1]};
+            r.ec_r = (uint32_t)(dr(hi) / 3 - level);
         }
-        if (i == 0) ec[0] = 1u;
     }
+    *out = r;
     return true;
 }
 
@@ -156,11 +185,9 @@ TC_DEV bool brt_node(View codes, uint32_t nk, uint32_t i, uint8_t* __restrict__ 
 // x < 2^(B - 1 - t).  With CHECK the probe also tests j < nk (windows shared by
 // several clouds); without it, window slots outside [0, nk) must hold an
 // all-ones sentinel, whose XOR with any code is at least 2^B, i.e. "no common
-// prefix".  Returns false, writing nothing, when the range reaches LIMIT.
+// prefix".  Returns false when the range reaches LIMIT.
 template <typename K, uint32_t LIMIT, bool CHECK>
-TC_DEV bool brt_node_near(lds_ptr<K> w, int32_t w0, uint32_t nk, uint32_t i, uint8_t* __restrict__ prefix_n,
-                          uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr, int32_t* __restrict__ left_child,
-                          int32_t* __restrict__ parent, uint32_t* __restrict__ ec) {
+TC_DEV bool brt_node_near(lds_ptr<K> w, int32_t w0, uint32_t nk, uint32_t i, bool want_ec, BrtResult* out) {
     constexpr int B = CodeTraits<K>::BITS;
     auto code = [&](uint32_t j) { return w[(int32_t)j - w0]; };
     auto dlt = [&](K a, K b) {  // -1 for "outside" (sentinel)
@@ -191,7 +218,6 @@ TC_DEV bool brt_node_near(lds_ptr<K> w, int32_t w0, uint32_t nk, uint32_t i, uin
         if (probe(l + t, tmin)) l += t;
     const uint32_t j = at(l);
     const int dnode = dlt(ci, code(j));
-    prefix_n[i] = (uint8_t)dnode;
     const K tnode = thr(dnode);
     uint32_t s = 0;
     const int kmax = l <= 1 ? 0 : log2_ceil_u32(l);
@@ -199,22 +225,19 @@ TC_DEV bool brt_node_near(lds_ptr<K> w, int32_t w0, uint32_t nk, uint32_t i, uin
         const uint32_t t = (l + (1u << k) - 1) >> k;
         if (probe(s + t, tnode)) s += t;
     }
-    const uint32_t gamma = fwd ? i + s : i - s - 1;
-    left_child[i] = (int32_t)gamma;
+    BrtResult r;
+    r.dnode = dnode;
+    r.gamma = fwd ? i + s : i - s - 1;
     const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
-    const bool leaf_l = lo == gamma;
-    const bool leaf_r = hi == gamma + 1;
-    hll[i] = leaf_l;
-    hlr[i] = leaf_r;
-    if (!leaf_l) parent[gamma] = (int32_t)i;
-    if (!leaf_r) parent[gamma + 1] = (int32_t)i;
-    if (i == 0) parent[0] = 0;
-    if (ec) {
+    r.leaf_l = lo == r.gamma;
+    r.leaf_r = hi == r.gamma + 1;
+    r.ec_l = r.ec_r = 0;
+    if (want_ec) {
         const int level = dnode / 3;
-        if (!leaf_l) ec[gamma] = (uint32_t)(dlt(code(lo), code(gamma)) / 3 - level);
-        if (!leaf_r) ec[gamma + 1] = (uint32_t)(dlt(code(gamma + 1), code(hi)) / 3 - level);
-        if (i == 0) ec[0] = 1u;
+        if (!r.leaf_l) r.ec_l = (uint32_t)(dlt(code(lo), code(r.gamma)) / 3 - level);
+        if (!r.leaf_r) r.ec_r = (uint32_t)(dlt(code(r.gamma + 1), code(hi)) / 3 - level);
     }
+    *out = r;
     return true;
 }
 
@@ -245,7 +268,7 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
                               const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ec,
                               const K* __restrict__ codes, const uint8_t* __restrict__ prefix_n,
                               const LinkView& lv, uint32_t a, uint32_t nb, int root_level, int link_leaves,
-                              float min_coord, float range) {
+                              float min_coord, float range, uint32_t write_base = 0) {
     constexpr int BITS = CodeTraits<K>::BITS;
     const uint32_t cnt = ec[a];
     if (cnt == 0) return;
@@ -315,9 +338,11 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
         for (int q = 0; q < 8; ++q) bch[q] = (slot == (uint32_t)q) ? v : bch[q];
     }
 
+    // Node idx (global) is stored at nodes[idx - write_base] (nodes may be one
+    // cloud's own buffer, of node_cap entries).
     for (uint32_t j = 0; j < cnt; ++j) {
         const uint32_t idx = off + j;
-        if (idx >= node_cap) break;
+        if (idx - write_base >= node_cap) break;
         const int level = bottom - (int)j;
         const int sh = BITS - 3 * level;
         float c4[4];
@@ -337,7 +362,7 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
             for (int q = 0; q < 8; ++q) ch[q] = (slot == (uint32_t)q) ? v : 0;
             mask = 1 << slot;
         }
-        int4* dst = reinterpret_cast<int4*>(nodes + idx);
+        int4* dst = reinterpret_cast<int4*>(nodes + (idx - write_base));
         dst[0] = make_int4(ch[0], ch[1], ch[2], ch[3]);
         dst[1] = make_int4(ch[4], ch[5], ch[6], ch[7]);
         dst[2] = make_int4(__float_as_int(c4[0]), __float_as_int(c4[1]), __float_as_int(c4[2]), __float_as_int(c4[3]));

@@ -318,10 +318,10 @@ int tc_batch_destroy(tc_batch* batch);
 int tc_batch_set(tc_batch* batch, uint32_t n_clouds, const uint32_t* counts, const float* mins,
                  const float* ranges, tc_stream stream);
 /* As tc_batch_set, for clouds held in separate device buffers: cloud c is read
- * from cloud_pts[c] (16-byte aligned xyzw), and when outs is non-NULL its
- * outputs are also copied into outs[c] at the end of the build (the
- * concatenated arrays stay valid as well).  Build or capture with
- * pts_xyzw = NULL after this call. */
+ * from cloud_pts[c] (16-byte aligned xyzw), and when outs is non-NULL the
+ * kernels that produce each output also store cloud c's copy in outs[c] (its
+ * octree nodes only there; the other concatenated arrays stay valid).  Build
+ * or capture with pts_xyzw = NULL after this call. */
 int tc_batch_set_io(tc_batch* batch, uint32_t n_clouds, const uint32_t* counts, const float* mins,
                     const float* ranges, const float* const* cloud_pts, const tc_tree_ptrs* outs,
                     tc_stream stream);
