@@ -302,7 +302,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* _
             s_c0 = seg_find(ubase, C, (uint32_t)t0);
             s_nlong = 0;
         }
-        for (int64_t j = w0 + threadIdx.x; j < w1; j += SEG_BLOCK) s_win[j - w0] = uniq[j];
+        stage_codes<SEG_BLOCK, SEG_BRT_TILE + 2 * SEG_SEARCH_HALO>(s_win, uniq, w0, w0, w1, (K)~(K)0);
         __syncthreads();
         const uint32_t c0 = s_c0;
 #pragma unroll 1
@@ -365,10 +365,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
         const int64_t w1 = t0 + SEG_BRT_TILE + SEG_BRT_HALO < total ? t0 + SEG_BRT_TILE + SEG_BRT_HALO : total;
         __syncthreads();
         if (threadIdx.x == 0) s_c0 = seg_find(ubase, C, (uint32_t)t0);
-        for (int64_t j = w0 + threadIdx.x; j < w1; j += SEG_BLOCK) {
-            s_left[j - w0] = left[j];
-            s_meta[j - w0] = LinkView::pack(ec[j], hll[j], hlr[j]);
-        }
+        stage_links<SEG_BLOCK, SEG_BRT_TILE + 2 * SEG_BRT_HALO>(s_left, s_meta, left, ec, hll, hlr, w0, w1);
         __syncthreads();
         const uint32_t c0 = s_c0;
         const bool to_clouds = (p.desc[3] & PLAN_OUT_PTRS) != 0;

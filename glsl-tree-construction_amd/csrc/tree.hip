@@ -57,10 +57,7 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
         const int64_t ws = t0 - BRT_HALO;
         __syncthreads();  // the previous tile is done with the window
         if (threadIdx.x == 0) s_nlong = 0;
-        for (int q = threadIdx.x; q < BRT_TILE + 2 * BRT_HALO; q += TREE_BLOCK) {
-            const int64_t j = ws + q;
-            s_win[q] = (j >= 0 && j < nk) ? codes[j] : (K)~(K)0;
-        }
+        stage_codes<TREE_BLOCK, BRT_TILE + 2 * BRT_HALO>(s_win, codes, ws, 0, nk, (K)~(K)0);
         __syncthreads();
 #pragma unroll 1
         for (int k = 0; k < BRT_TILE / TREE_BLOCK; ++k) {
@@ -120,10 +117,7 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_octree_build(
         const int64_t w0 = t0 > OCT_HALO ? t0 - OCT_HALO : 0;
         const int64_t w1 = t0 + OCT_TILE + OCT_HALO < nb ? t0 + OCT_TILE + OCT_HALO : nb;
         __syncthreads();
-        for (int64_t j = w0 + threadIdx.x; j < w1; j += TREE_BLOCK) {
-            s_left[j - w0] = left_child[j];
-            s_meta[j - w0] = LinkView::pack(ec[j], hll[j], hlr[j]);
-        }
+        stage_links<TREE_BLOCK, OCT_TILE + 2 * OCT_HALO>(s_left, s_meta, left_child, ec, hll, hlr, w0, w1);
         __syncthreads();
         // Only nodes with ec > 0 emit anything (typically well under half):
         // compact them so that every lane of a wave has one.

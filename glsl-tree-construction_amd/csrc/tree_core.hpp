@@ -64,6 +64,61 @@ struct LinkView {
     }
 };
 
+// Window staging.  Thread t takes entries t, t + BLOCK, ... of the W-entry
+// window starting at j0; all of its loads are issued into registers before the
+// first LDS store, so a tile pays one memory round trip for its window (a
+// load-then-store loop waits once per iteration).  Indexes are clamped into
+// the valid range (which must not be empty) instead of branching around the
+// loads; entries outside [lo, hi) then get `fill`.
+template <int BLOCK, int W, typename K>
+TC_DEV void stage_codes(K* dst, const K* __restrict__ src, int64_t j0, int64_t lo, int64_t hi, K fill) {
+    constexpr int PER = (W + BLOCK - 1) / BLOCK;
+    K v[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int64_t j = j0 + r * BLOCK + (int)threadIdx.x;
+        v[r] = src[j < lo ? lo : (j < hi ? j : hi - 1)];
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int64_t j = j0 + r * BLOCK + (int)threadIdx.x;
+        if (j < lo || j >= hi) v[r] = fill;
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int q = r * BLOCK + (int)threadIdx.x;
+        if (q < W) dst[q] = v[r];
+    }
+}
+
+// Links of entries [w0, w1) (0 < w1 - w0 <= W) packed for LinkView.
+template <int BLOCK, int W>
+TC_DEV void stage_links(int32_t* s_left, uint16_t* s_meta, const int32_t* __restrict__ left,
+                        const uint32_t* __restrict__ ec, const uint8_t* __restrict__ hll,
+                        const uint8_t* __restrict__ hlr, int64_t w0, int64_t w1) {
+    constexpr int PER = (W + BLOCK - 1) / BLOCK;
+    int32_t vl[PER];
+    uint32_t ve[PER];
+    uint8_t va[PER], vb[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        int64_t j = w0 + r * BLOCK + (int)threadIdx.x;
+        j = j < w1 ? j : w1 - 1;
+        vl[r] = left[j];
+        ve[r] = ec[j];
+        va[r] = hll[j];
+        vb[r] = hlr[j];
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int q = r * BLOCK + (int)threadIdx.x;
+        if (w0 + q < w1) {
+            s_left[q] = vl[r];
+            s_meta[q] = LinkView::pack(ve[r], va[r], vb[r]);
+        }
+    }
+}
+
 // Common-prefix length of codes[i] and codes[j] over the code width; -1 for
 // j outside [0, nk) (j is taken modulo 2^32, so i - x for x > i is outside).
 template <typename K, typename View = CodeView<K>>
