@@ -12,6 +12,8 @@ Also reported (same JSON line):
   roofline      dominant kernel (the binning pass): algorithmic bytes / average
                 launch time, launch times from HIP events around each launch on
                 its own stream (outside the timed loop)
+  indexed_build the headline build with the point -> leaf map (point index
+                sorted along with the codes, run starts from unique)
   sort          standalone u32 key sort of the same codes (histogram + 4
                 passes) against the 36 B/key algorithmic traffic
   sort_pairs_u64  2^27 63-bit keys with u32 payloads, 8 passes (200 B/pair)
@@ -424,6 +426,17 @@ def main():
     }
     tree.free()
     if not args.no_extras:
+        # Same build with the point -> leaf map (index payload through the sort,
+        # run starts from unique).
+        itree = tc.Tree(ctx, n, 30, point_index=True)
+        ims = timed(lambda: itree.build(pts, n, 0.0, 1024.0), max(2, args.steps // 2), 1) * 1e3
+        itree.free()
+        out["indexed_build"] = {
+            "workload": "the headline build plus the point -> leaf map (u32 point index sorted with the codes, "
+                        "run starts from unique)",
+            "ms": round(ims, 4),
+            "Mpoints_per_s": round(n / (ims * 1e-3) / 1e6, 2),
+        }
         codes = torch.empty(n, dtype=torch.int32, device="cuda")
         ctx.morton30(pts, codes, 0.0, 1024.0)
         out["sort"] = bench_sort_u32(ctx, codes, args.profile_steps)

@@ -178,14 +178,8 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
             const v4 qv = src[i - c0];
             const float4 q = make_float4(qv.x, qv.y, qv.z, qv.w);
             K code;
-            if constexpr (sizeof(K) == 4) {
-                code = d_split3_10(d_quantize(q.x, mn, rg, 1024.0f)) | (d_split3_10(d_quantize(q.y, mn, rg, 1024.0f)) << 1) |
-                       (d_split3_10(d_quantize(q.z, mn, rg, 1024.0f)) << 2);
-            } else {
-                code = d_split3_21(d_quantize(q.x, mn, rg, 2097152.0f)) |
-                       (d_split3_21(d_quantize(q.y, mn, rg, 2097152.0f)) << 1) |
-                       (d_split3_21(d_quantize(q.z, mn, rg, 2097152.0f)) << 2);
-            }
+            if constexpr (sizeof(K) == 4) code = d_encode30(q.x, q.y, q.z, mn, rg);
+            else code = d_encode63(q.x, q.y, q.z, mn, rg);
             codes[i] = code;
             lds_count_digits<K, DIG>(s_h, code);
         }
@@ -436,6 +430,7 @@ int seg_pass(tc_batch* b, hipStream_t s, const K* src, K* dst, uint32_t pass) {
     TC_SEG_CASE(512, 12)
     TC_SEG_CASE(512, 16)
     TC_SEG_CASE(512, 20)
+    TC_SEG_CASE(512, 24)
 #undef TC_SEG_CASE
     return TC_ERR_INVALID;
 }

@@ -163,9 +163,12 @@ extern "C" int tc_create(tc_ctx** out, int device) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cus = prop.multiProcessorCount;
     const size_t status_bytes = ((size_t)(1u << 18) + 1) * sizeof(uint64_t);
+    const size_t acc_bytes = (size_t)HIST_COPIES * 8 * 256 * 4;
     if (hipMalloc(&ctx->small, SMALL_WORDS * 4) != hipSuccess ||
         hipMalloc(&ctx->scan_status, status_bytes) != hipSuccess ||
+        hipMalloc(&ctx->hist_acc, acc_bytes) != hipSuccess ||
         hipMemset(ctx->small, 0, SMALL_WORDS * 4) != hipSuccess ||
+        hipMemset(ctx->hist_acc, 0, acc_bytes) != hipSuccess ||
         hipMemset(ctx->scan_status, 0, status_bytes) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
         tc_destroy(ctx);
@@ -183,6 +186,7 @@ extern "C" int tc_destroy(tc_ctx* ctx) {
     for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->small) (void)hipFree(ctx->small);
+    if (ctx->hist_acc) (void)hipFree(ctx->hist_acc);
     if (ctx->scan_status) (void)hipFree(ctx->scan_status);
     delete ctx;
     return TC_OK;
@@ -200,8 +204,9 @@ extern "C" int tc_set_sort_params(tc_ctx* ctx, int block, int kpt) {
     if (!ctx) return TC_ERR_INVALID;
     if (block == 0) block = ctx->sort_block;
     if (kpt == 0) kpt = ctx->sort_kpt;
-    if (block != 256 && block != 512) return TC_ERR_INVALID;
-    if (kpt != 8 && kpt != 12 && kpt != 16 && kpt != 20) return TC_ERR_INVALID;
+    bool built = false;  // one of the shapes the binning pass is compiled for
+    for (const SortShape& sh : SORT_SHAPES) built = built || (sh.block == block && sh.kpt == kpt);
+    if (!built) return TC_ERR_INVALID;
     ctx->sort_block = block;
     ctx->sort_kpt = kpt;
     ctx->tune_on = false;  // an explicit shape replaces tuned ones
@@ -271,7 +276,12 @@ extern "C" void tc_morton64_to_xyz(float ret[4], uint64_t code, float min_coord,
 
 // ------------------------------------------------------------ plain pipeline
 extern "C" int tc_tree_alloc(tc_ctx* ctx, uint32_t capacity, int code_bits, tc_tree* t) {
+    return tc_tree_alloc_ex(ctx, capacity, code_bits, 0, t);
+}
+
+extern "C" int tc_tree_alloc_ex(tc_ctx* ctx, uint32_t capacity, int code_bits, int flags, tc_tree* t) {
     if (!ctx || !t || (code_bits != 30 && code_bits != 63)) return TC_ERR_INVALID;
+    if (flags & ~TC_TREE_POINT_INDEX) return TC_ERR_INVALID;
     if (capacity > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
     TC_TRY(set_device(ctx));
     std::memset(t, 0, sizeof(*t));
@@ -293,6 +303,10 @@ extern "C" int tc_tree_alloc(tc_ctx* ctx, uint32_t capacity, int code_bits, tc_t
               hipMalloc((void**)&t->nodes, std::max<uint64_t>(t->node_capacity, 1) * sizeof(tc_oct_node)) ==
                   hipSuccess &&
               hipMalloc((void**)&t->d_counts, 16) == hipSuccess;
+    if (ok && (flags & TC_TREE_POINT_INDEX))
+        ok = hipMalloc((void**)&t->point_index, n * 4) == hipSuccess &&
+             hipMalloc((void**)&t->index_alt, n * 4) == hipSuccess &&
+             hipMalloc((void**)&t->leaf_first, (n + 1) * 4) == hipSuccess;
     if (!ok) {
         tc_tree_free(ctx, t);
         return TC_ERR_NOMEM;
@@ -304,7 +318,8 @@ extern "C" int tc_tree_free(tc_ctx* ctx, tc_tree* t) {
     if (!ctx || !t) return TC_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
     void* ptrs[] = {t->sorted, t->alt, t->unique, t->prefix_n, t->has_leaf_left, t->has_leaf_right,
-                    t->left_child, t->parent, t->edge_count, t->offsets, t->nodes, t->d_counts};
+                    t->left_child, t->parent, t->edge_count, t->offsets, t->nodes, t->d_counts,
+                    t->point_index, t->index_alt, t->leaf_first};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     std::memset(t, 0, sizeof(*t));
@@ -324,11 +339,20 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     // morton -> sort -> unique (count stays on the device) -> radix tree with
     // fused edge counts -> exclusive scan -> octree.  Stages after unique read
     // the unique count from d_counts; the scan always writes offsets[n_brt].
+    // With a point index the sort carries each point's index as its payload and
+    // unique also records where each code's run starts (the leaf -> points map).
+    const bool indexed = t->point_index != nullptr;
     if (t->code_bits == 30) {
         uint32_t* sorted = static_cast<uint32_t*>(t->sorted);
         uint32_t* uniq = static_cast<uint32_t*>(t->unique);
-        TC_TRY(tc_morton_sort_u32(ctx, pts, sorted, static_cast<uint32_t*>(t->alt), n, min_coord, range, stream));
-        TC_TRY(tc_unique_u32(ctx, sorted, uniq, n, t->d_counts, stream));
+        if (indexed) {
+            TC_TRY(tc_morton_sort_pairs_u32(ctx, pts, sorted, t->point_index, static_cast<uint32_t*>(t->alt),
+                                            t->index_alt, n, min_coord, range, stream));
+            TC_TRY(tc_unique_runs_u32(ctx, sorted, uniq, t->leaf_first, n, t->d_counts, stream));
+        } else {
+            TC_TRY(tc_morton_sort_u32(ctx, pts, sorted, static_cast<uint32_t*>(t->alt), n, min_coord, range, stream));
+            TC_TRY(tc_unique_u32(ctx, sorted, uniq, n, t->d_counts, stream));
+        }
         TC_TRY(tc_build_radix_tree_ec_u32(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
                                           t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
         TC_TRY(tc_exclusive_scan_u32(ctx, t->edge_count, t->offsets, n - 1, t->d_counts, stream));
@@ -339,8 +363,14 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     } else {
         uint64_t* sorted = static_cast<uint64_t*>(t->sorted);
         uint64_t* uniq = static_cast<uint64_t*>(t->unique);
-        TC_TRY(tc_morton_sort_u64(ctx, pts, sorted, static_cast<uint64_t*>(t->alt), n, min_coord, range, stream));
-        TC_TRY(tc_unique_u64(ctx, sorted, uniq, n, t->d_counts, stream));
+        if (indexed) {
+            TC_TRY(tc_morton_sort_pairs_u64(ctx, pts, sorted, t->point_index, static_cast<uint64_t*>(t->alt),
+                                            t->index_alt, n, min_coord, range, stream));
+            TC_TRY(tc_unique_runs_u64(ctx, sorted, uniq, t->leaf_first, n, t->d_counts, stream));
+        } else {
+            TC_TRY(tc_morton_sort_u64(ctx, pts, sorted, static_cast<uint64_t*>(t->alt), n, min_coord, range, stream));
+            TC_TRY(tc_unique_u64(ctx, sorted, uniq, n, t->d_counts, stream));
+        }
         TC_TRY(tc_build_radix_tree_ec_u64(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
                                           t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
         TC_TRY(tc_exclusive_scan_u32(ctx, t->edge_count, t->offsets, n - 1, t->d_counts, stream));

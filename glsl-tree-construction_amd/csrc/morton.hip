@@ -16,9 +16,7 @@ __global__ __launch_bounds__(MORTON_BLOCK) void k_morton30(const float4* __restr
     const uint32_t stride = gridDim.x * MORTON_BLOCK;
     for (uint32_t i = blockIdx.x * MORTON_BLOCK + threadIdx.x; i < n; i += stride) {
         const float4 p = pts[i];
-        codes[i] = d_split3_10(d_quantize(p.x, min_coord, range, 1024.0f)) |
-                   (d_split3_10(d_quantize(p.y, min_coord, range, 1024.0f)) << 1) |
-                   (d_split3_10(d_quantize(p.z, min_coord, range, 1024.0f)) << 2);
+        codes[i] = d_encode30(p.x, p.y, p.z, min_coord, range);
     }
 }
 
@@ -28,9 +26,7 @@ __global__ __launch_bounds__(MORTON_BLOCK) void k_morton63(const float4* __restr
     const uint32_t stride = gridDim.x * MORTON_BLOCK;
     for (uint32_t i = blockIdx.x * MORTON_BLOCK + threadIdx.x; i < n; i += stride) {
         const float4 p = pts[i];
-        codes[i] = d_split3_21(d_quantize(p.x, min_coord, range, 2097152.0f)) |
-                   (d_split3_21(d_quantize(p.y, min_coord, range, 2097152.0f)) << 1) |
-                   (d_split3_21(d_quantize(p.z, min_coord, range, 2097152.0f)) << 2);
+        codes[i] = d_encode63(p.x, p.y, p.z, min_coord, range);
     }
 }
 

@@ -8,7 +8,7 @@
 // shader/edge_count.comp:30-41.  Published algorithm: Merrill & Garland,
 // "Single-pass Parallel Prefix Scan with Decoupled Look-back" (2016).
 //
-// gfx950 design: one 4096-element tile per workgroup, tile = workgroup id (a
+// gfx950 design: one 8192-element tile per workgroup, tile = workgroup id (a
 // claim counter would serialise at about 11 ns per claim on MI355X).  A tile is
 // loaded with 16 B vector loads (each wave instruction reads 1 KiB contiguous)
 // into LDS and read back so each thread owns 16 consecutive elements; a block
@@ -30,7 +30,7 @@
 
 namespace {
 
-constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_BLOCK = 512;
 constexpr int SCAN_IPT = 16;
 constexpr uint32_t SCAN_TILE = SCAN_BLOCK * SCAN_IPT;
 
@@ -349,6 +349,21 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         __syncthreads();
         T* out = static_cast<T*>(a.out0);
         for (uint32_t k = tid; k < block_total; k += SCAN_BLOCK) out[tile_excl + k] = s_items[k];
+        if (a.out1) {
+            // Run starts: the position of each kept key (its first occurrence),
+            // compacted the same way.
+            uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_items);
+            run = texcl;
+            __syncthreads();  // the keys are out of LDS
+#pragma unroll
+            for (int j = 0; j < SCAN_IPT; ++j) {
+                if (val[j]) s_pos[run] = i0 + j;
+                run += val[j];
+            }
+            __syncthreads();
+            uint32_t* first = static_cast<uint32_t*>(a.out1);
+            for (uint32_t k = tid; k < block_total; k += SCAN_BLOCK) first[tile_excl + k] = s_pos[k];
+        }
     } else {
         uint32_t run = tile_excl + texcl;
         uint32_t* ec = static_cast<uint32_t*>(a.out0);
@@ -366,6 +381,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         const uint32_t total = tile_excl + block_total;
         if (is_plain_values(MODE)) static_cast<uint32_t*>(a.out0)[n] = total;
         if (MODE == MODE_ECSCAN) static_cast<uint32_t*>(a.out1)[n] = total;
+        if (is_unique(MODE) && a.out1) static_cast<uint32_t*>(a.out1)[total] = n;  // end of the last run
         if (a.count_out) *a.count_out = total;
     }
 }
@@ -390,7 +406,7 @@ int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void
                    uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s) {
     const uint32_t tiles = ceil_div_u32(n ? n : 1, SCAN_TILE);
     if (!ctx->scan_status) return TC_ERR_NOMEM;
-    // Status words for up to 2^30 / 4096 tiles are preallocated with the context.
+    // Status words for 2^18 tiles (more than 2^30 / SCAN_TILE) are preallocated with the context.
     if (tiles > (1u << 18)) return TC_ERR_TOO_LARGE;
     ScanCtl* ctl = reinterpret_cast<ScanCtl*>(ctx->small + SMALL_SCAN_CTR);
     ScanArgs a{in0, in1, out0, out1, count_out, n, d_n_keys, seg_off, n_seg_cap, d_n_seg, seg_out,
@@ -425,6 +441,22 @@ extern "C" int tc_unique_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out,
     if (!ctx || !d_n_unique || (n && (!sorted || !out))) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
     return tc_launch_scan(ctx, MODE_UNIQUE64, sorted, nullptr, out, nullptr, d_n_unique, n, nullptr, nullptr, 0,
+                          nullptr, nullptr, as_stream(stream));
+}
+
+extern "C" int tc_unique_runs_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out, uint32_t* run_first,
+                                  uint32_t n, uint32_t* d_n_unique, tc_stream stream) {
+    if (!ctx || !d_n_unique || !run_first || (n && (!sorted || !out))) return TC_ERR_INVALID;
+    if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    return tc_launch_scan(ctx, MODE_UNIQUE32, sorted, nullptr, out, run_first, d_n_unique, n, nullptr, nullptr, 0,
+                          nullptr, nullptr, as_stream(stream));
+}
+
+extern "C" int tc_unique_runs_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out, uint32_t* run_first,
+                                  uint32_t n, uint32_t* d_n_unique, tc_stream stream) {
+    if (!ctx || !d_n_unique || !run_first || (n && (!sorted || !out))) return TC_ERR_INVALID;
+    if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    return tc_launch_scan(ctx, MODE_UNIQUE64, sorted, nullptr, out, run_first, d_n_unique, n, nullptr, nullptr, 0,
                           nullptr, nullptr, as_stream(stream));
 }
 

@@ -8,10 +8,11 @@
 //
 // gfx950 design:
 //  * histogram: one read of the keys (16 B vector loads) builds all 4 (u32) or
-//    8 (u64) digit histograms in LDS sub-histograms, merged with device atomics;
-//    the last block to finish turns the counts into exclusive prefixes and
-//    clears the accumulator for the next call.  It also clears the pass-0
-//    look-back words and the per-pass tile counters.
+//    8 (u64) digit histograms in LDS sub-histograms, merged with device atomics
+//    into 32 accumulator copies (same-address device atomics serialise); a
+//    one-workgroup-per-digit kernel sums the copies into exclusive prefixes and
+//    clears them for the next call.  The histogram kernel also clears the
+//    pass-0 look-back words and the per-pass tile counters.
 //    In the pipeline the histogram is fused into the Morton kernel instead.
 //  * binning pass: a tile of BLOCK*KPT keys per workgroup (tile = workgroup
 //    id).  Each wave ranks its 64*KPT keys with 64-lane ballots (8 ballots give
@@ -32,6 +33,7 @@ namespace {
 constexpr int HIST_BLOCK = 256;
 constexpr int HIST_REP = 4;  // LDS sub-histogram copies per block (one per wave)
 constexpr int HG_ITEMS = 4;  // 16-byte key loads in flight per thread in the histogram kernel
+static_assert(HIST_BLOCK == 256, "one thread per bin in k_hist_finish");
 
 template <typename K> struct KeyVec;
 template <> struct KeyVec<uint32_t> { using V = uint4; static constexpr int N = 4; };
@@ -48,44 +50,47 @@ TC_DEV void vec_get<uint64_t>(const ulonglong2& v, uint64_t* out) {
     out[0] = v.x; out[1] = v.y;
 }
 
-// Merge a block's LDS sub-histograms into the device accumulator; the last
-// block to finish turns the counts into exclusive prefixes (digit-major) and
-// clears the accumulator and the ticket for the next call.
+// Merge a block's LDS sub-histograms into accumulator copy blockIdx % HIST_COPIES
+// (ctx->hist_acc, HIST_COPIES x 8 x 256 words).  Device atomics to one address
+// serialise (one per block and bin would make a chain of gridDim.x adds per
+// bin at the end of the kernel); spread over the copies the chains are
+// gridDim.x / HIST_COPIES long.  k_hist_finish sums the copies.
 template <int DIG>
-TC_DEV void hist_finish(const uint32_t* s_h, uint32_t* s_tmp, uint32_t* s_last, uint32_t* __restrict__ acc,
-                        uint32_t* __restrict__ done, uint32_t* __restrict__ hist_out) {
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t b = tid; b < DIG * 256; b += HIST_BLOCK) {
-        uint32_t s = 0;
+TC_DEV void hist_merge(const uint32_t* s_h, uint32_t* __restrict__ acc) {
+    uint32_t* a = acc + (blockIdx.x % HIST_COPIES) * (8 * 256);
+    for (uint32_t b = threadIdx.x; b < DIG * 256; b += HIST_BLOCK) {
+        uint32_t v = 0;
 #pragma unroll
-        for (int r = 0; r < HIST_REP; ++r) s += s_h[r * DIG * 256 + b];
-        if (s) atomicAdd(&acc[b], s);
+        for (int r = 0; r < HIST_REP; ++r) v += s_h[r * DIG * 256 + b];
+        if (v) atomicAdd(&a[b], v);
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) *s_last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1u : 0u;
-    __syncthreads();
-    if (!*s_last) return;
-    __threadfence();
-    for (int d = 0; d < DIG; ++d) {
-        const uint32_t c = ld_acquire_agent(&acc[d * 256 + tid]);
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<HIST_BLOCK>(c, s_tmp, &tot);
-        hist_out[d * 256 + tid] = ex;
-        st_agent(&acc[d * 256 + tid], 0u);
+}
+
+// One workgroup per digit: the digit's counts summed over the accumulator
+// copies, written as an exclusive prefix (digit-major, the layout of
+// shader/histogram.comp:103-195); the copies are cleared for the next use.
+__global__ __launch_bounds__(HIST_BLOCK) void k_hist_finish(uint32_t* __restrict__ acc,
+                                                            uint32_t* __restrict__ hist_out) {
+    __shared__ uint32_t s_tmp[HIST_BLOCK / 64 + 2];
+    const uint32_t d = blockIdx.x, tid = threadIdx.x;
+    uint32_t c = 0;
+#pragma unroll 8
+    for (int r = 0; r < HIST_COPIES; ++r) {
+        uint32_t* p = &acc[r * (8 * 256) + d * 256 + tid];
+        c += *p;
+        *p = 0u;
     }
-    if (tid == 0) st_agent(done, 0u);
+    uint32_t tot;
+    hist_out[d * 256 + tid] = block_exclusive_scan<HIST_BLOCK>(c, s_tmp, &tot);
 }
 
 template <typename K>
 __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
-    const K* __restrict__ keys, uint32_t n, uint32_t* __restrict__ acc, uint32_t* __restrict__ done,
-    uint32_t* __restrict__ hist_out, uint32_t* __restrict__ zero_ptr, uint32_t zero_words) {
+    const K* __restrict__ keys, uint32_t n, uint32_t* __restrict__ acc, uint32_t* __restrict__ zero_ptr,
+    uint32_t zero_words) {
     constexpr int DIG = (int)sizeof(K);
     constexpr int VN = KeyVec<K>::N;
     __shared__ uint32_t s_h[HIST_REP][DIG * 256];
-    __shared__ uint32_t s_tmp[HIST_BLOCK / 64 + 2];
-    __shared__ uint32_t s_last;
 
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < HIST_REP * DIG * 256; i += HIST_BLOCK) (&s_h[0][0])[i] = 0;
@@ -126,40 +131,30 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
     // Clear the pass-0 look-back words and the tile counters for the passes.
     for (uint32_t w = gtid; w < zero_words; w += gstride) zero_ptr[w] = 0u;
     __syncthreads();
-    hist_finish<DIG>(&s_h[0][0], s_tmp, &s_last, acc, done, hist_out);
+    hist_merge<DIG>(&s_h[0][0], acc);
 }
 
 template <typename K>
 TC_DEV K encode_point(const float4& p, float min_coord, float range) {
-    if constexpr (sizeof(K) == 4) {
-        return d_split3_10(d_quantize(p.x, min_coord, range, 1024.0f)) |
-               (d_split3_10(d_quantize(p.y, min_coord, range, 1024.0f)) << 1) |
-               (d_split3_10(d_quantize(p.z, min_coord, range, 1024.0f)) << 2);
-    } else {
-        return d_split3_21(d_quantize(p.x, min_coord, range, 2097152.0f)) |
-               (d_split3_21(d_quantize(p.y, min_coord, range, 2097152.0f)) << 1) |
-               (d_split3_21(d_quantize(p.z, min_coord, range, 2097152.0f)) << 2);
-    }
+    if constexpr (sizeof(K) == 4) return d_encode30(p.x, p.y, p.z, min_coord, range);
+    else return d_encode63(p.x, p.y, p.z, min_coord, range);
 }
 
 // Morton codes with the sort's global histogram fused in: the codes are counted
 // while they are in registers, so the sort does not read them an extra time
 // (k_ComputeMortonCode, shader/morton.comp:60-67 + k_GlobalHistogram,
-// shader/histogram.comp:103-195).  The grid is small (the histogram merge costs
-// one device atomic per bin per block), so each thread keeps MH_ITEMS point
-// loads in flight to cover HBM latency.
+// shader/histogram.comp:103-195).  Each thread keeps MH_ITEMS point loads in
+// flight to cover HBM latency.
 constexpr int MH_ITEMS = 8;
 
 template <typename K>
 __global__ __launch_bounds__(HIST_BLOCK) void k_morton_hist(
-    const float4* __restrict__ pts, K* __restrict__ codes, uint32_t n, float min_coord, float range,
-    uint32_t* __restrict__ acc, uint32_t* __restrict__ done, uint32_t* __restrict__ hist_out,
-    uint32_t* __restrict__ zero_ptr, uint32_t zero_words) {
+    const float4* __restrict__ pts, K* __restrict__ codes, uint32_t* __restrict__ index, uint32_t n,
+    float min_coord, float range, uint32_t* __restrict__ acc, uint32_t* __restrict__ zero_ptr,
+    uint32_t zero_words) {
     constexpr int DIG = (int)sizeof(K);
     constexpr uint32_t CHUNK = HIST_BLOCK * MH_ITEMS;
     __shared__ uint32_t s_h[HIST_REP][DIG * 256];
-    __shared__ uint32_t s_tmp[HIST_BLOCK / 64 + 2];
-    __shared__ uint32_t s_last;
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < HIST_REP * DIG * 256; i += HIST_BLOCK) (&s_h[0][0])[i] = 0;
     __syncthreads();
@@ -174,6 +169,7 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_morton_hist(
         for (int u = 0; u < MH_ITEMS; ++u) {
             const K code = encode_point<K>(p[u], min_coord, range);
             codes[b0 + u * HIST_BLOCK] = code;
+            if (index) index[b0 + u * HIST_BLOCK] = b0 + u * HIST_BLOCK;  // sort payload: the point's index
             lds_count_digits<K, DIG>(h, code);
         }
     }
@@ -182,11 +178,12 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_morton_hist(
     for (uint32_t i = full * CHUNK + gtid; i < n; i += gstride) {
         const K code = encode_point<K>(pts[i], min_coord, range);
         codes[i] = code;
+        if (index) index[i] = i;
         lds_count_digits<K, DIG>(h, code);
     }
     for (uint32_t w = gtid; w < zero_words; w += gstride) zero_ptr[w] = 0u;
     __syncthreads();
-    hist_finish<DIG>(&s_h[0][0], s_tmp, &s_last, acc, done, hist_out);
+    hist_merge<DIG>(&s_h[0][0], acc);
 }
 
 template <typename K, bool PAIRS, int BLOCK, int KPT>
@@ -237,13 +234,16 @@ int dispatch_pass(tc_ctx* ctx, SortShape sh, uint32_t tiles, hipStream_t s, cons
     TC_PASS_CASE(512, 12)
     TC_PASS_CASE(512, 16)
     TC_PASS_CASE(512, 20)
+    TC_PASS_CASE(512, 24)
 #undef TC_PASS_CASE
     return TC_ERR_INVALID;
 }
 
+// Eight workgroups (32 waves) per CU: enough loads in flight to stream at
+// HBM speed; the merge's atomic chains stay gridDim / HIST_COPIES long.
 uint32_t hist_grid(tc_ctx* ctx, uint32_t n) {
     uint32_t blocks = ceil_div_u32(n, HIST_BLOCK * 16);
-    uint32_t cap = (uint32_t)ctx->num_cus * 2;
+    uint32_t cap = (uint32_t)ctx->num_cus * 8;
     if (blocks < 1) blocks = 1;
     return blocks < cap ? blocks : cap;
 }
@@ -275,14 +275,17 @@ int histogram_impl(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist_out, u
                    uint32_t zero_words, hipStream_t s) {
     ProfScope prof(ctx, s, K_HIST);
     hipLaunchKernelGGL((k_global_histogram<K>), dim3(hist_grid(ctx, n)), dim3(HIST_BLOCK), 0, s, keys, n,
-                       ctx->small + SMALL_HIST_ACC, ctx->small + SMALL_HIST_DONE, hist_out, zero_ptr,
-                       zero_words);
+                       ctx->hist_acc, zero_ptr, zero_words);
+    TC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_hist_finish, dim3(sizeof(K)), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, hist_out);
     TC_LAUNCH_CHECK();
     return TC_OK;
 }
 
 // Sort keys (and values) in place.  With pts != NULL the keys are first
-// computed from the points by the fused Morton + histogram kernel.
+// computed from the points by the fused Morton + histogram kernel, which also
+// sets vals[i] = i when vals is given (the sorted values are then the
+// permutation from sorted position to point index).
 template <typename K>
 int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint32_t n, hipStream_t s,
               const float* pts = nullptr, float min_coord = 0.f, float range = 1.f) {
@@ -301,9 +304,10 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
     if (pts) {
         ProfScope prof(ctx, s, K_MORTON);
         hipLaunchKernelGGL((k_morton_hist<K>), dim3(hist_grid(ctx, n)), dim3(HIST_BLOCK), 0, s,
-                           reinterpret_cast<const float4*>(pts), keys, n, min_coord, range,
-                           ctx->small + SMALL_HIST_ACC, ctx->small + SMALL_HIST_DONE, ss.hist, ss.ctr,
-                           64 + region_words);
+                           reinterpret_cast<const float4*>(pts), keys, vals, n, min_coord, range, ctx->hist_acc,
+                           ss.ctr, 64 + region_words);
+        TC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_hist_finish, dim3(DIG), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, ss.hist);
         TC_LAUNCH_CHECK();
     } else {
         TC_TRY(histogram_impl<K>(ctx, keys, n, ss.hist, ss.ctr, 64 + region_words, s));
@@ -393,6 +397,18 @@ extern "C" int tc_morton_sort_u32(tc_ctx* ctx, const float* pts, uint32_t* keys,
                                   float min_coord, float range, tc_stream stream) {
     if (n && (!pts || (reinterpret_cast<uintptr_t>(pts) & 15u))) return TC_ERR_INVALID;
     return sort_impl<uint32_t>(ctx, keys, nullptr, alt, nullptr, n, as_stream(stream), pts, min_coord, range);
+}
+extern "C" int tc_morton_sort_pairs_u32(tc_ctx* ctx, const float* pts, uint32_t* keys, uint32_t* index,
+                                        uint32_t* keys_alt, uint32_t* index_alt, uint32_t n, float min_coord,
+                                        float range, tc_stream stream) {
+    if (n && (!pts || !index || (reinterpret_cast<uintptr_t>(pts) & 15u))) return TC_ERR_INVALID;
+    return sort_impl<uint32_t>(ctx, keys, index, keys_alt, index_alt, n, as_stream(stream), pts, min_coord, range);
+}
+extern "C" int tc_morton_sort_pairs_u64(tc_ctx* ctx, const float* pts, uint64_t* keys, uint32_t* index,
+                                        uint64_t* keys_alt, uint32_t* index_alt, uint32_t n, float min_coord,
+                                        float range, tc_stream stream) {
+    if (n && (!pts || !index || (reinterpret_cast<uintptr_t>(pts) & 15u))) return TC_ERR_INVALID;
+    return sort_impl<uint64_t>(ctx, keys, index, keys_alt, index_alt, n, as_stream(stream), pts, min_coord, range);
 }
 extern "C" int tc_morton_sort_u64(tc_ctx* ctx, const float* pts, uint64_t* keys, uint64_t* alt, uint32_t n,
                                   float min_coord, float range, tc_stream stream) {

@@ -85,16 +85,20 @@ struct tc_ctx {
     // Permanent small device block: histogram accumulators (self-cleaning),
     // completion counters, pipeline counters.
     uint32_t* small = nullptr;
-    // Scan tile status words (epoch-tagged, never cleared): one per 4096-element
+    // Sort histogram accumulators: HIST_COPIES x 8 x 256 counters, zero
+    // between uses (the finishing kernel clears what it reads).
+    uint32_t* hist_acc = nullptr;
+    // Scan tile status words (epoch-tagged, never cleared): one per 8192-element
     // tile up to 2^30 elements.
     uint64_t* scan_status = nullptr;
     // Host-side staging for the host-array wrappers.
     std::mutex mu;
 };
 
-// Layout of ctx->small (u32 words).
-constexpr size_t SMALL_HIST_ACC = 0;        // 8*256 digit counters, zero between uses
-constexpr size_t SMALL_HIST_DONE = 2048;    // completion ticket of the histogram
+// Copies of the sort histogram accumulator (ctx->hist_acc).
+constexpr int HIST_COPIES = 32;
+
+// Layout of ctx->small (u32 words; words 0-2055 unused).
 constexpr size_t SMALL_TILE_CTR = 2056;     // 8 per-pass tile counters (+ 8 spare)
 constexpr size_t SMALL_SCAN_CTR = 2072;     // scan tile counters (8)
 constexpr size_t SMALL_WORDS = 4096;
@@ -108,9 +112,9 @@ size_t tc_sort_scratch_bytes(uint32_t n, int passes, int tile);
 struct SortShape {
     int block, kpt;
 };
-constexpr int N_SORT_SHAPES = 8;
-constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {{256, 8}, {256, 12}, {256, 16}, {256, 20},
-                                                  {512, 8}, {512, 12}, {512, 16}, {512, 20}};
+constexpr int N_SORT_SHAPES = 9;
+constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {{256, 8}, {256, 12}, {256, 16}, {256, 20}, {512, 8},
+                                                  {512, 12}, {512, 16}, {512, 20}, {512, 24}};
 // The shape a sort of n keys uses: the tuned one for the nearest measured size
 // when the context has been tuned, else the context's fixed shape.
 SortShape tc_sort_shape(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n);
@@ -193,16 +197,19 @@ TC_DEV uint32_t mbcnt64(uint64_t mask) {
 // Adds the DIG 8-bit digits of `key` to the LDS histograms h[d * 256 + digit].
 // When every active lane of the wave holds the same digit (the high digits of
 // sorted or spatially coherent input), one lane adds the wave's count: 64
-// same-address LDS atomics would otherwise serialise.
+// same-address LDS atomics would otherwise serialise.  The branch on that test
+// is wave-uniform; the leader and the count are computed once per key.
 template <typename K, int DIG>
 TC_DEV void lds_count_digits(uint32_t* h, K key) {
     const uint64_t active = __ballot(true);
+    const bool leader = mbcnt64(active) == 0u;
+    const uint32_t n_active = (uint32_t)__popcll(active);
 #pragma unroll
     for (int d = 0; d < DIG; ++d) {
         const uint32_t v = (uint32_t)((key >> (8 * d)) & 255u);
         const uint32_t v0 = __builtin_amdgcn_readfirstlane(v);
         if (__ballot(v == v0) == active) {
-            if (mbcnt64(active) == 0u) atomicAdd(&h[d * 256 + v0], (uint32_t)__popcll(active));
+            if (leader) atomicAdd(&h[d * 256 + v0], n_active);
         } else {
             atomicAdd(&h[d * 256 + v], 1u);
         }
@@ -296,19 +303,51 @@ TC_DEV uint64_t d_compact3_21(uint64_t m) {
 
 // GLSL uint(float) with out-of-range input made well defined: NaN and
 // negatives give 0, values >= 2^32 give 0xFFFFFFFF, otherwise truncation.
+// Branch-free: v_max_f32 returns the non-NaN operand, so NaN becomes 0.
 TC_DEV uint32_t d_f2u_sat(float t) {
-    if (!(t > 0.0f)) return 0u;
-    if (t >= 4294967296.0f) return 0xFFFFFFFFu;
-    return (uint32_t)t;
+    const float c = fmaxf(t, 0.0f);
+    return c >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)c;
 }
 
+// scale / range when it is an exact power of two that is a normal fp32 value,
+// else 0.  For such a ratio, (t / range) * scale and t * (scale / range) give
+// the same quantised integer for every t: dividing by a power of two and
+// multiplying by one are exact unless the quotient is subnormal, and a
+// subnormal quotient times scale (<= 2^21) is below 1, which both forms
+// truncate to 0 (an overflow gives +inf in both).  `scale` is a power of two.
+TC_DEV float d_pow2_ratio(float range, float scale) {
+    const uint32_t rb = __float_as_uint(range);
+    const int re = (int)((rb >> 23) & 255u);
+    if ((rb & 0x807FFFFFu) != 0u || re == 0 || re == 255) return 0.0f;  // not a positive normal power of two
+    const int se = (int)((__float_as_uint(scale) >> 23) & 255u);
+    const int e = se - re + 127;  // biased exponent of scale / range
+    return (e >= 1 && e <= 254) ? __uint_as_float((uint32_t)e << 23) : 0.0f;
+}
+
+// The Morton quantiser of one cloud: per-axis integer grid coordinate
 // ((v - min) / range) * scale with separately rounded fp32 steps
-// (single_point_to_code_v2, shader/morton.comp:50-56).
-TC_DEV uint32_t d_quantize(float v, float min_coord, float range, float scale) {
-    float t = __fsub_rn(v, min_coord);
-    t = __fdiv_rn(t, range);
-    t = __fmul_rn(t, scale);
-    return d_f2u_sat(t);
+// (single_point_to_code_v2, shader/morton.comp:50-56), with one multiply in
+// place of the division and the scaling when scale / range is an exact power
+// of two (every range 2^k).
+struct MortonQuant {
+    float min_coord, range, scale, ratio;
+    TC_DEV MortonQuant(float mn, float rg, float sc) : min_coord(mn), range(rg), scale(sc), ratio(d_pow2_ratio(rg, sc)) {}
+    TC_DEV uint32_t operator()(float v) const {
+        const float t = __fsub_rn(v, min_coord);
+        if (ratio != 0.0f) return d_f2u_sat(__fmul_rn(t, ratio));
+        return d_f2u_sat(__fmul_rn(__fdiv_rn(t, range), scale));
+    }
+};
+
+// 30-bit code of a point (k_ComputeMortonCode, shader/morton.comp:60-67).
+TC_DEV uint32_t d_encode30(float x, float y, float z, float min_coord, float range) {
+    const MortonQuant q(min_coord, range, 1024.0f);
+    return d_split3_10(q(x)) | (d_split3_10(q(y)) << 1) | (d_split3_10(q(z)) << 2);
+}
+// 63-bit code: the same quantiser with a 2^21 grid.
+TC_DEV uint64_t d_encode63(float x, float y, float z, float min_coord, float range) {
+    const MortonQuant q(min_coord, range, 2097152.0f);
+    return d_split3_21(q(x)) | (d_split3_21(q(y)) << 1) | (d_split3_21(q(z)) << 2);
 }
 
 // Code traits: 30-bit codes in u32, 63-bit codes in u64.

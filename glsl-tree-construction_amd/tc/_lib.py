@@ -64,7 +64,13 @@ class TcTree(C.Structure):
         ("n_unique", C.c_uint32),
         ("n_brt", C.c_uint32),
         ("n_nodes", C.c_uint32),
+        ("point_index", C.c_void_p),
+        ("index_alt", C.c_void_p),
+        ("leaf_first", C.c_void_p),
     ]
+
+
+TREE_POINT_INDEX = 1  # tc_tree_alloc_ex flag
 
 
 CLOUD_INFO_DTYPE = np.dtype(
@@ -205,8 +211,12 @@ SIGNATURES = {
     "tc_morton_sort_u32": (I, [P, P, P, P, U32, F, F, S]),
     "tc_morton_sort_u64": (I, [P, P, P, P, U32, F, F, S]),
     "tc_sort_pairs_u64": (I, [P, P, P, P, P, U32, S]),
+    "tc_morton_sort_pairs_u32": (I, [P, P, P, P, P, P, U32, F, F, S]),
+    "tc_morton_sort_pairs_u64": (I, [P, P, P, P, P, P, U32, F, F, S]),
     "tc_unique_u32": (I, [P, P, P, U32, P, S]),
     "tc_unique_u64": (I, [P, P, P, U32, P, S]),
+    "tc_unique_runs_u32": (I, [P, P, P, P, U32, P, S]),
+    "tc_unique_runs_u64": (I, [P, P, P, P, U32, P, S]),
     "tc_build_radix_tree_u32": (I, [P, P, U32, P, P, P, P, P, P, S]),
     "tc_build_radix_tree_u64": (I, [P, P, U32, P, P, P, P, P, P, S]),
     "tc_build_radix_tree_ec_u32": (I, [P, P, U32, P, P, P, P, P, P, P, S]),
@@ -218,6 +228,7 @@ SIGNATURES = {
     "tc_build_octree_u64": (I, [P, P, U32, P, P, P, P, P, P, P, P, U32, P, F, F, I, S]),
     "tc_octree_node_bound": (U64, [U64, I]),
     "tc_tree_alloc": (I, [P, U32, I, C.POINTER(TcTree)]),
+    "tc_tree_alloc_ex": (I, [P, U32, I, I, C.POINTER(TcTree)]),
     "tc_tree_free": (I, [P, C.POINTER(TcTree)]),
     "tc_build_tree": (I, [P, P, U32, F, F, I, C.POINTER(TcTree), S]),
     "tc_tree_read_counts": (I, [P, C.POINTER(TcTree), S]),

@@ -20,7 +20,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import OCT_NODE_DTYPE, TcError, TcTree, check, lib
+from ._lib import OCT_NODE_DTYPE, TREE_POINT_INDEX, TcError, TcTree, check, lib
 
 _default_ctx = None
 
@@ -148,10 +148,24 @@ class Context:
                                          _stream(stream))
         check(st, "tc_sort")
 
+    def morton_sort_pairs(self, pts, keys, index, keys_alt, index_alt, min_coord, range_, n=None, stream=None):
+        """As morton_sort, with each point's input index sorted along (int32 ``index``)."""
+        n = pts.shape[0] if n is None else n
+        fn = lib().tc_morton_sort_pairs_u64 if keys.element_size() == 8 else lib().tc_morton_sort_pairs_u32
+        check(fn(self._h, _t_ptr(pts), _t_ptr(keys), _t_ptr(index), _t_ptr(keys_alt), _t_ptr(index_alt), n,
+                 min_coord, range_, _stream(stream)), "tc_morton_sort_pairs")
+
     def unique(self, sorted_keys, out, d_count, n=None, stream=None):
         n = sorted_keys.shape[0] if n is None else n
         fn = lib().tc_unique_u64 if sorted_keys.element_size() == 8 else lib().tc_unique_u32
         check(fn(self._h, _t_ptr(sorted_keys), _t_ptr(out), n, _t_ptr(d_count), _stream(stream)), "tc_unique")
+
+    def unique_runs(self, sorted_keys, out, run_first, d_count, n=None, stream=None):
+        """unique plus run starts: run_first[j] = position of run j's first key, run_first[U] = n."""
+        n = sorted_keys.shape[0] if n is None else n
+        fn = lib().tc_unique_runs_u64 if sorted_keys.element_size() == 8 else lib().tc_unique_runs_u32
+        check(fn(self._h, _t_ptr(sorted_keys), _t_ptr(out), _t_ptr(run_first), n, _t_ptr(d_count),
+                 _stream(stream)), "tc_unique_runs")
 
     def build_radix_tree(self, codes, n_keys, prefix_n, hll, hlr, left, parent, d_n_keys=None, stream=None):
         fn = lib().tc_build_radix_tree_u64 if codes.element_size() == 8 else lib().tc_build_radix_tree_u32
@@ -182,10 +196,14 @@ class Context:
 class Tree:
     """Device-resident buffers of one plain build (tc_tree)."""
 
-    def __init__(self, ctx: Context, capacity: int, code_bits: int = 30):
+    def __init__(self, ctx: Context, capacity: int, code_bits: int = 30, point_index: bool = False):
+        """point_index=True also builds the point -> leaf map on every build:
+        ``point_index`` (sorted position -> input point) and ``leaf_first``
+        (leaf j holds point_index[leaf_first[j]:leaf_first[j+1]])."""
         self.ctx = ctx
         self.t = TcTree()
-        check(lib().tc_tree_alloc(ctx.handle, capacity, code_bits, C.byref(self.t)), "tc_tree_alloc")
+        flags = TREE_POINT_INDEX if point_index else 0
+        check(lib().tc_tree_alloc_ex(ctx.handle, capacity, code_bits, flags, C.byref(self.t)), "tc_tree_alloc_ex")
 
     def free(self):
         if self.t.capacity or self.t.sorted:
@@ -233,6 +251,9 @@ class Tree:
         res["offsets"] = dl(self.t.offsets, nb + 1, np.uint32) if n else np.zeros(1, np.uint32)
         raw = dl(self.t.nodes, no * 64, np.uint8)
         res["nodes"] = raw.view(OCT_NODE_DTYPE)
+        if self.t.point_index:
+            res["point_index"] = dl(self.t.point_index, n, np.uint32)
+            res["leaf_first"] = dl(self.t.leaf_first, u + 1, np.uint32) if n else np.zeros(1, np.uint32)
         return res
 
 

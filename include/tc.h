@@ -73,8 +73,9 @@ int tc_destroy(tc_ctx* ctx);
 /* Pre-size scratch for sorts/scans of up to max_n elements so that later
  * calls allocate nothing (needed before stream capture). */
 int tc_reserve(tc_ctx* ctx, uint32_t max_n);
-/* Launch parameters of the sort's binning pass: block size (256 or 512) and
- * keys per thread (8, 12, 16 or 20).  0 keeps the current value. */
+/* Launch parameters of the sort's binning pass, one of the compiled shapes
+ * (block, keys per thread): 256 x {8, 12, 16, 20} and 512 x {8, 12, 16, 20,
+ * 24}.  0 keeps the current value. */
 int tc_set_sort_params(tc_ctx* ctx, int block, int keys_per_thread);
 int tc_get_sort_params(tc_ctx* ctx, int* block, int* keys_per_thread);
 /* Autotuner (no reference counterpart): times every binning-pass launch shape
@@ -159,6 +160,16 @@ int tc_morton_sort_u32(tc_ctx* ctx, const float* pts_xyzw, uint32_t* keys, uint3
                        float min_coord, float range, tc_stream stream);
 int tc_morton_sort_u64(tc_ctx* ctx, const float* pts_xyzw, uint64_t* keys, uint64_t* keys_alt, uint32_t n,
                        float min_coord, float range, tc_stream stream);
+/* Morton codes sorted together with each point's index (values start as
+ * 0..n-1 and travel with the stable sort): keys[k] is the k-th smallest code
+ * and index[k] the input position of its point.  No reference counterpart
+ * (the reference sorts codes only, src/main.rs:85); the point -> leaf map. */
+int tc_morton_sort_pairs_u32(tc_ctx* ctx, const float* pts_xyzw, uint32_t* keys, uint32_t* index,
+                             uint32_t* keys_alt, uint32_t* index_alt, uint32_t n, float min_coord,
+                             float range, tc_stream stream);
+int tc_morton_sort_pairs_u64(tc_ctx* ctx, const float* pts_xyzw, uint64_t* keys, uint32_t* index,
+                             uint64_t* keys_alt, uint32_t* index_alt, uint32_t n, float min_coord,
+                             float range, tc_stream stream);
 int tc_sort_pairs_u64(tc_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                       uint32_t* vals_alt, uint32_t n, tc_stream stream);
 
@@ -169,6 +180,16 @@ int tc_unique_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out, uint32_t n
                   uint32_t* d_n_unique, tc_stream stream);
 int tc_unique_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out, uint32_t n,
                   uint32_t* d_n_unique, tc_stream stream);
+/* Unique with run starts (the same std::unique semantics as above, plus
+ * run_first[j] = sorted position of the first key of run j and
+ * run_first[U] = n; run_first has n + 1 entries).  Run j of sorted codes is
+ * leaf j of the radix tree, so with the sort permutation this maps each leaf
+ * to its points.  No reference counterpart (the reference keeps only the
+ * codes, src/unique.rs:278-283). */
+int tc_unique_runs_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out, uint32_t* run_first,
+                       uint32_t n, uint32_t* d_n_unique, tc_stream stream);
+int tc_unique_runs_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out, uint32_t* run_first,
+                       uint32_t n, uint32_t* d_n_unique, tc_stream stream);
 
 /* ---- Karras binary radix tree: k_BuildRadixTree,
  *      shader/build_radix_tree.comp:71-153; host wrapper build_radix_tree,
@@ -255,9 +276,20 @@ typedef struct {
     uint32_t* d_counts;       /* device [4]: n_unique, n_brt, n_nodes, reserved */
     /* filled by tc_tree_read_counts */
     uint32_t n_points, n_unique, n_brt, n_nodes;
+    /* Point -> leaf map (NULL unless allocated with TC_TREE_POINT_INDEX).  No
+     * reference counterpart: the reference sorts codes only (src/main.rs:85).
+     * Leaf j (unique code j) holds the points point_index[leaf_first[j] ..
+     * leaf_first[j+1]); point_index is the stable sort permutation (sorted
+     * position -> input index) and leaf_first[n_unique] = n_points. */
+    uint32_t* point_index;    /* [capacity] */
+    uint32_t* index_alt;      /* [capacity] sort ping-pong space for the index */
+    uint32_t* leaf_first;     /* [capacity+1] */
 } tc_tree;
 
+#define TC_TREE_POINT_INDEX 1 /* tc_tree_alloc_ex flag: also build the point -> leaf map */
+
 int tc_tree_alloc(tc_ctx* ctx, uint32_t capacity, int code_bits, tc_tree* tree);
+int tc_tree_alloc_ex(tc_ctx* ctx, uint32_t capacity, int code_bits, int flags, tc_tree* tree);
 int tc_tree_free(tc_ctx* ctx, tc_tree* tree);
 int tc_build_tree(tc_ctx* ctx, const float* pts_xyzw, uint32_t n, float min_coord, float range,
                   int link_leaves, tc_tree* tree, tc_stream stream);

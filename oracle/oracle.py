@@ -61,6 +61,8 @@ def lib():
             "tco_sort_u64": (None, [P, P, sz, P, P]),
             "tco_unique_u32": (u32, [P, P, sz]),
             "tco_unique_u64": (u32, [P, P, sz]),
+            "tco_unique_runs_u32": (u32, [P, P, P, sz]),
+            "tco_unique_runs_u64": (u32, [P, P, P, sz]),
             "tco_build_radix_tree_u32": (None, [P, u32, P, P, P, P, P]),
             "tco_build_radix_tree_u64": (None, [P, u32, P, P, P, P, P]),
             "tco_edge_count": (None, [P, P, P, u32, i32]),
@@ -160,6 +162,16 @@ def unique(sorted_keys: np.ndarray) -> np.ndarray:
     return out[:n].copy()
 
 
+def unique_runs(sorted_keys: np.ndarray):
+    """(unique keys, run starts with the total appended)."""
+    s = np.ascontiguousarray(sorted_keys)
+    out = np.empty_like(s)
+    first = np.empty(len(s) + 1, np.uint32)
+    fn = lib().tco_unique_runs_u32 if s.dtype == np.uint32 else lib().tco_unique_runs_u64
+    n = fn(_p(s), _p(out), _p(first), len(s))
+    return out[:n].copy(), first[:n + 1].copy()
+
+
 # ------------------------------------------------------------ Radix tree
 def build_radix_tree(codes: np.ndarray):
     c = np.ascontiguousarray(codes)
@@ -209,14 +221,24 @@ def octree_node_bound(n_keys: int, code_bits: int) -> int:
 
 
 # ------------------------------------------------------- Whole pipeline
-def build_tree(points: np.ndarray, min_coord: float, range_: float, code_bits: int = 30, link_leaves=True):
+def build_tree(points: np.ndarray, min_coord: float, range_: float, code_bits: int = 30, link_leaves=True,
+               point_index=False):
     """morton -> sort -> unique -> radix tree -> edge count -> scan -> octree,
-    the stage order of src/main.rs:60-170."""
+    the stage order of src/main.rs:60-170.  point_index=True sorts each
+    point's index along (stable pairs sort) and keeps the run starts of unique:
+    the point -> leaf map (point_index, leaf_first)."""
     codes = compute_morton(points, min_coord, range_) if code_bits == 30 else compute_morton63(points, min_coord, range_)
-    sorted_codes = radix_sort(codes)
-    uniq = unique(sorted_codes)
+    extra = {}
+    if point_index:
+        sorted_codes, perm = radix_sort(codes, np.arange(len(codes), dtype=np.uint32))
+        uniq, first = unique_runs(sorted_codes)
+        extra = dict(point_index=perm, leaf_first=first)
+    else:
+        sorted_codes = radix_sort(codes)
+        uniq = unique(sorted_codes)
     tree = build_radix_tree(uniq)
     ec = edge_count(tree["prefix_n"], tree["parent"], code_bits)
     offsets = exclusive_scan(ec)
     nodes = build_octree(offsets, ec, uniq, tree, min_coord, range_, link_leaves)
-    return dict(codes=codes, sorted=sorted_codes, unique=uniq, edge_count=ec, offsets=offsets, nodes=nodes, **tree)
+    return dict(codes=codes, sorted=sorted_codes, unique=uniq, edge_count=ec, offsets=offsets, nodes=nodes, **tree,
+                **extra)

@@ -234,6 +234,41 @@ uint32_t tco_unique_u64(const uint64_t* s, uint64_t* out, size_t n) {
     return (uint32_t)(j + 1);
 }
 
+/* Unique with run starts: the std::unique loop above, also recording the
+ * position of each run's first key (first[j]) and first[U] = n.  No reference
+ * counterpart (the reference keeps only the codes, src/unique.rs:278-283). */
+uint32_t tco_unique_runs_u32(const uint32_t* s, uint32_t* out, uint32_t* first, size_t n) {
+    size_t j = 0;
+    if (n) {
+        out[0] = s[0];
+        first[0] = 0;
+        for (size_t i = 1; i < n; ++i)
+            if (s[i] != out[j]) {
+                out[++j] = s[i];
+                first[j] = (uint32_t)i;
+            }
+        ++j;
+    }
+    first[j] = (uint32_t)n;
+    return (uint32_t)j;
+}
+
+uint32_t tco_unique_runs_u64(const uint64_t* s, uint64_t* out, uint32_t* first, size_t n) {
+    size_t j = 0;
+    if (n) {
+        out[0] = s[0];
+        first[0] = 0;
+        for (size_t i = 1; i < n; ++i)
+            if (s[i] != out[j]) {
+                out[++j] = s[i];
+                first[j] = (uint32_t)i;
+            }
+        ++j;
+    }
+    first[j] = (uint32_t)n;
+    return (uint32_t)j;
+}
+
 /* ------------------------------------------------------------- Radix tree */
 
 static int clz32_(uint32_t x) { return x ? __builtin_clz(x) : 32; }

@@ -60,6 +60,8 @@ void tco_sort_u64(uint64_t* keys, uint32_t* values, size_t n, uint64_t* tmp_k, u
 /* ---- Unique (shader/unique.comp:31-45): out[0..U) = first of each run. */
 uint32_t tco_unique_u32(const uint32_t* sorted, uint32_t* out, size_t n);
 uint32_t tco_unique_u64(const uint64_t* sorted, uint64_t* out, size_t n);
+uint32_t tco_unique_runs_u32(const uint32_t* sorted, uint32_t* out, uint32_t* first, size_t n);
+uint32_t tco_unique_runs_u64(const uint64_t* sorted, uint64_t* out, uint32_t* first, size_t n);
 
 /* ---- Karras binary radix tree (shader/build_radix_tree.comp:44-153).
  * n_keys unique sorted codes -> n_keys-1 internal nodes. code_bits = 30 or 63. */

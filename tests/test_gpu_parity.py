@@ -114,6 +114,31 @@ def test_out_of_range_inputs_match_oracle(gpu_ctx, oracle):
     assert np.array_equal(host(c63, np.uint64), oracle.compute_morton63(pts, 0.0, 1024.0))
 
 
+@pytest.mark.parametrize("min_coord,range_", [(0.0, 2.0), (-3.25, 0.5), (100.0, 2097152.0), (-1e-3, 1.0 / 1024),
+                                              (7.0, 4096.0), (0.0, 3.0)])
+def test_pow2_ranges_match_division(gpu_ctx, oracle, min_coord, range_):
+    """With range a power of two the kernels multiply by scale / range; the
+    oracle divides.  Random points, exact grid boundaries, their fp32
+    neighbours and out-of-range values must quantise identically."""
+    rng = np.random.default_rng(17)
+    n = 20_000
+    span = np.float32(range_)
+    lo = np.float32(min_coord)
+    x = (rng.random((n, 3), dtype=np.float32) * np.float32(1.2) - np.float32(0.1)) * span + lo
+    g = np.arange(0, 1025, dtype=np.float32) / np.float32(1024) * span + lo  # grid boundaries
+    edges = np.concatenate([g, np.nextafter(g, np.float32(-np.inf)), np.nextafter(g, np.float32(np.inf)),
+                            np.array([np.nan, np.inf, -np.inf, 1e-45, -1e-45, 1e38], np.float32)])
+    e = np.resize(edges, (len(edges) // 3 + 1) * 3).reshape(-1, 3)
+    pts = np.concatenate([x, e]).astype(np.float32)
+    pts = np.concatenate([pts, np.ones((len(pts), 1), np.float32)], axis=1)
+    codes = empty(len(pts), np.uint32)
+    gpu_ctx.morton30(dev(pts), codes, min_coord, range_)
+    assert np.array_equal(host(codes, np.uint32), oracle.compute_morton(pts, min_coord, range_))
+    c63 = empty(len(pts), np.uint64)
+    gpu_ctx.morton63(dev(pts), c63, min_coord, range_)
+    assert np.array_equal(host(c63, np.uint64), oracle.compute_morton63(pts, min_coord, range_))
+
+
 def test_nonunit_min_range(gpu_ctx, oracle, tclib):
     pts = tclib.gen_uniform(50_000, 31, 1.0)
     pts[:, :3] = pts[:, :3] * np.float32(37.0) - np.float32(11.0)
@@ -131,7 +156,7 @@ def test_decode_kernel(gpu_ctx, oracle):
 
 
 @pytest.mark.parametrize("block,kpt", [(256, 8), (256, 12), (256, 16), (256, 20), (512, 8), (512, 12),
-                                       (512, 16), (512, 20)])
+                                       (512, 16), (512, 20), (512, 24)])
 def test_sort_launch_shapes(gpu_ctx, oracle, block, kpt):
     """Every binning launch shape the tuner may pick gives the stable order."""
     r = np.random.default_rng(block * 100 + kpt)

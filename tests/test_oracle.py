@@ -76,6 +76,33 @@ def test_quantize_fp32_order_of_operations(oracle):
     assert int((wide != got).sum()) > 100
 
 
+def test_pow2_range_multiply_equals_division():
+    """The kernels quantise with t * (scale / range) when that ratio is an exact
+    power of two; the reference divides and then scales.  Over random,
+    boundary, subnormal, huge and non-finite t, in IEEE fp32, both give the
+    same saturated integer for every power-of-two range and both grid scales."""
+    f = np.float32
+    r = rng(23)
+    t = np.concatenate([
+        (r.random(200_000) * 4 - 1).astype(f) * f(2.0) ** r.integers(-30, 30, 200_000).astype(f),
+        np.array([0.0, -0.0, 1e-45, -1e-45, 1.17e-38, 3.4e38, -3.4e38, np.inf, -np.inf, np.nan], f),
+        np.frombuffer(r.integers(0, 1 << 32, 200_000, dtype=np.uint64).astype(np.uint32).tobytes(), f),
+    ]).astype(f)
+
+    def sat(x):
+        with np.errstate(invalid="ignore"):
+            c = np.where(np.isnan(x), f(0), np.maximum(x, f(0)))
+            return np.where(c >= f(4294967296.0), np.uint64(0xFFFFFFFF), c.astype(np.float64).astype(np.uint64))
+
+    with np.errstate(over="ignore", under="ignore", invalid="ignore"):
+        for scale in (f(1024.0), f(2097152.0)):
+            for e in range(-40, 41, 3):
+                rg = f(2.0) ** f(e)
+                div = sat((t / rg) * scale)
+                mul = sat(t * (scale / rg))
+                assert np.array_equal(div, mul), (scale, e)
+
+
 def test_quantize_out_of_range_is_defined(oracle):
     q = oracle.lib().tco_quantize
     assert q(float("nan"), 0.0, 1024.0, 1024.0) == 0
@@ -202,6 +229,40 @@ def test_unique_vs_numpy(oracle):
         assert np.array_equal(oracle.unique(keys), np.unique(keys))
     k = np.full(1000, 7, np.uint32)
     assert oracle.unique(k).tolist() == [7]
+
+
+def test_unique_runs_vs_numpy(oracle):
+    """Run starts are numpy's first-occurrence indices of the sorted keys, with n appended."""
+    r = rng(19)
+    for n in (0, 1, 2, 5, 10_000):
+        for dt in (np.uint32, np.uint64):
+            keys = np.sort(r.integers(0, 300, size=n, dtype=np.uint64).astype(dt))
+            u, first = oracle.unique_runs(keys)
+            want_u, want_first = np.unique(keys, return_index=True)
+            assert np.array_equal(u, want_u)
+            assert np.array_equal(first[:-1], want_first) and first[-1] == n
+    u, first = oracle.unique_runs(np.full(1000, 7, np.uint32))
+    assert u.tolist() == [7] and first.tolist() == [0, 1000]
+
+
+def test_point_index_map(oracle, tclib):
+    """The indexed build's permutation is the stable sort of the codes, and
+    leaf j's points are exactly the input points whose code is unique[j]."""
+    pts = tclib.gen_gaussian_mixture(30_000, 5, 8, 0.5)  # heavy duplicates at 30 bits
+    for bits in (30, 63):
+        t = oracle.build_tree(pts, 0.0, 1024.0, bits, point_index=True)
+        plain = oracle.build_tree(pts, 0.0, 1024.0, bits)
+        for k in ("sorted", "unique", "prefix_n", "left_child", "parent", "offsets"):
+            assert np.array_equal(t[k], plain[k]), k
+        perm, first, codes = t["point_index"], t["leaf_first"], t["codes"]
+        assert np.array_equal(perm, np.argsort(codes, kind="stable").astype(np.uint32))
+        assert np.array_equal(codes[perm], t["sorted"])
+        assert first[0] == 0 and first[-1] == len(pts) and np.all(np.diff(first.astype(np.int64)) > 0)
+        sizes = np.diff(first.astype(np.int64))
+        leaf_of_point = np.repeat(np.arange(len(t["unique"])), sizes)
+        assert np.array_equal(t["unique"][leaf_of_point], codes[perm])
+        if bits == 30:
+            assert len(t["unique"]) < len(pts) // 4
 
 
 # ------------------------------------------------------------- Radix tree
