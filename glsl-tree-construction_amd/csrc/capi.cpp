@@ -81,6 +81,8 @@ extern "C" const char* tc_status_string(int status) {
         case TC_ERR_BUSY: return "busy";
         case TC_ERR_NOT_FOUND: return "not found";
         case TC_ERR_NO_DEVICE: return "no usable GPU";
+        case TC_ERR_BAD_TREE: return "tree invariant violated";
+        case TC_ERR_IO: return "file I/O error";
         default: return "unknown status";
     }
 }
@@ -332,6 +334,9 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     if (n > t->capacity) return TC_ERR_CAPACITY;
     hipStream_t s = as_stream(stream);
     t->n_points = n;
+    t->min_coord = min_coord;
+    t->range = range;
+    t->link_leaves = link_leaves ? 1 : 0;
     // Counts start at zero so that an empty cloud reads back as empty.
     TC_TRY(tc_fill_u32(ctx, t->d_counts, 0u, 4, s));
     if (n == 0) return TC_OK;
@@ -394,6 +399,81 @@ extern "C" int tc_tree_read_counts(tc_ctx* ctx, tc_tree* t, tc_stream stream) {
     if (t->n_brt > 0) TC_HIP(hipMemcpy(&t->n_nodes, t->offsets + t->n_brt, 4, hipMemcpyDeviceToHost));
     if (t->n_nodes > t->node_capacity) return TC_ERR_CAPACITY;
     return TC_OK;
+}
+
+// ------------------------------------------- tree download, check, export
+namespace {
+// Host copies of a device tree's arrays, described by a tc_tree_host.
+struct HostTree {
+    std::vector<uint8_t> sorted, unique, prefix_n, hll, hlr;
+    std::vector<int32_t> left, parent;
+    std::vector<uint32_t> ec, offsets, point_index, leaf_first;
+    std::vector<tc_oct_node> nodes;
+    tc_tree_host view{};
+};
+
+template <typename T>
+int d2h(std::vector<T>& v, const void* d, size_t count) {
+    v.resize(count);
+    if (count) TC_HIP(hipMemcpy(v.data(), d, count * sizeof(T), hipMemcpyDeviceToHost));
+    return TC_OK;
+}
+
+int download_tree(tc_ctx* ctx, tc_tree* t, tc_stream stream, HostTree* h) {
+    TC_TRY(tc_tree_read_counts(ctx, t, stream));
+    const size_t ks = t->code_bits == 30 ? 4 : 8;
+    const uint32_t n = t->n_points, u = t->n_unique, nb = t->n_brt;
+    TC_TRY(d2h(h->sorted, t->sorted, n * ks));
+    TC_TRY(d2h(h->unique, t->unique, u * ks));
+    TC_TRY(d2h(h->prefix_n, t->prefix_n, nb));
+    TC_TRY(d2h(h->hll, t->has_leaf_left, nb));
+    TC_TRY(d2h(h->hlr, t->has_leaf_right, nb));
+    TC_TRY(d2h(h->left, t->left_child, nb));
+    TC_TRY(d2h(h->parent, t->parent, nb));
+    TC_TRY(d2h(h->ec, t->edge_count, nb));
+    TC_TRY(d2h(h->offsets, t->offsets, nb ? nb + 1 : 0));
+    TC_TRY(d2h(h->nodes, t->nodes, t->n_nodes));
+    if (t->point_index) {
+        TC_TRY(d2h(h->point_index, t->point_index, n));
+        if (n) TC_TRY(d2h(h->leaf_first, t->leaf_first, u + 1));
+        else h->leaf_first.assign(1, 0u);
+    }
+    tc_tree_host& v = h->view;
+    v.code_bits = t->code_bits;
+    v.min_coord = t->min_coord;
+    v.range = t->range;
+    v.link_leaves = t->link_leaves;
+    v.n_points = n;
+    v.n_unique = u;
+    v.n_nodes = t->n_nodes;
+    v.sorted = h->sorted.data();
+    v.unique = h->unique.data();
+    v.prefix_n = h->prefix_n.data();
+    v.has_leaf_left = h->hll.data();
+    v.has_leaf_right = h->hlr.data();
+    v.left_child = h->left.data();
+    v.parent = h->parent.data();
+    v.edge_count = h->ec.data();
+    v.offsets = nb ? h->offsets.data() : nullptr;
+    v.nodes = h->nodes.data();
+    v.point_index = t->point_index ? h->point_index.data() : nullptr;
+    v.leaf_first = t->point_index ? h->leaf_first.data() : nullptr;
+    return TC_OK;
+}
+}  // namespace
+
+extern "C" int tc_tree_validate(tc_ctx* ctx, tc_tree* t, char* msg, size_t msg_len, tc_stream stream) {
+    if (!ctx || !t) return TC_ERR_INVALID;
+    HostTree h;
+    TC_TRY(download_tree(ctx, t, stream, &h));
+    return tc_tree_validate_host(&h.view, msg, msg_len);
+}
+
+extern "C" int tc_tree_export(tc_ctx* ctx, tc_tree* t, const char* path, tc_stream stream) {
+    if (!ctx || !t || !path) return TC_ERR_INVALID;
+    HostTree h;
+    TC_TRY(download_tree(ctx, t, stream, &h));
+    return tc_tree_write_host(&h.view, path);
 }
 
 // ------------------------------------------------------ host-array mirrors

@@ -26,3 +26,4 @@ from .api import (  # noqa: F401
 from .batch import Batch  # noqa: F401,E402
 from .serve import Server, plain_run_trace, result_bytes, simulate  # noqa: F401,E402
 from . import tune  # noqa: F401,E402
+from .treeio import load_tree, validate_tree, write_tree  # noqa: F401,E402

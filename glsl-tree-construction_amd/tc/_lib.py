@@ -18,10 +18,11 @@ TC_OK = 0
 STATUS_NAMES = {
     0: "TC_OK", -1: "TC_ERR_INVALID", -2: "TC_ERR_NOMEM", -3: "TC_ERR_TOO_LARGE", -4: "TC_ERR_HIP",
     -5: "TC_ERR_OVER_BUDGET", -6: "TC_ERR_CAPACITY", -7: "TC_ERR_BUSY", -8: "TC_ERR_NOT_FOUND",
-    -9: "TC_ERR_NO_DEVICE",
+    -9: "TC_ERR_NO_DEVICE", -10: "TC_ERR_BAD_TREE", -11: "TC_ERR_IO",
 }
 TC_ERR_INVALID, TC_ERR_NOMEM, TC_ERR_TOO_LARGE, TC_ERR_HIP = -1, -2, -3, -4
 TC_ERR_OVER_BUDGET, TC_ERR_CAPACITY, TC_ERR_BUSY, TC_ERR_NOT_FOUND, TC_ERR_NO_DEVICE = -5, -6, -7, -8, -9
+TC_ERR_BAD_TREE, TC_ERR_IO = -10, -11
 
 OCT_NODE_DTYPE = np.dtype(
     [
@@ -66,6 +67,34 @@ class TcTree(C.Structure):
         ("n_nodes", C.c_uint32),
         ("point_index", C.c_void_p),
         ("index_alt", C.c_void_p),
+        ("leaf_first", C.c_void_p),
+        ("min_coord", C.c_float),
+        ("range", C.c_float),
+        ("link_leaves", C.c_int),
+    ]
+
+
+class TcTreeHost(C.Structure):
+    """tc_tree_host: one tree in host memory (optional arrays may be NULL)."""
+    _fields_ = [
+        ("code_bits", C.c_int),
+        ("min_coord", C.c_float),
+        ("range", C.c_float),
+        ("link_leaves", C.c_int),
+        ("n_points", C.c_uint32),
+        ("n_unique", C.c_uint32),
+        ("n_nodes", C.c_uint32),
+        ("sorted", C.c_void_p),
+        ("unique", C.c_void_p),
+        ("prefix_n", C.c_void_p),
+        ("has_leaf_left", C.c_void_p),
+        ("has_leaf_right", C.c_void_p),
+        ("left_child", C.c_void_p),
+        ("parent", C.c_void_p),
+        ("edge_count", C.c_void_p),
+        ("offsets", C.c_void_p),
+        ("nodes", C.c_void_p),
+        ("point_index", C.c_void_p),
         ("leaf_first", C.c_void_p),
     ]
 
@@ -232,6 +261,10 @@ SIGNATURES = {
     "tc_tree_free": (I, [P, C.POINTER(TcTree)]),
     "tc_build_tree": (I, [P, P, U32, F, F, I, C.POINTER(TcTree), S]),
     "tc_tree_read_counts": (I, [P, C.POINTER(TcTree), S]),
+    "tc_tree_validate_host": (I, [C.POINTER(TcTreeHost), C.c_char_p, C.c_size_t]),
+    "tc_tree_write_host": (I, [C.POINTER(TcTreeHost), C.c_char_p]),
+    "tc_tree_validate": (I, [P, C.POINTER(TcTree), C.c_char_p, C.c_size_t, S]),
+    "tc_tree_export": (I, [P, C.POINTER(TcTree), C.c_char_p, S]),
     "tc_host_compute_morton": (I, [P, P, P, U32, F, F, U32]),
     "tc_host_histogram": (I, [P, U32, P, P]),
     "tc_host_radix_sort": (I, [P, P, U32, P, U32]),

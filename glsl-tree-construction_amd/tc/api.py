@@ -219,6 +219,19 @@ class Tree:
         check(lib().tc_build_tree(self.ctx.handle, _t_ptr(pts), n, min_coord, range_, int(bool(link_leaves)),
                                   C.byref(self.t), _stream(stream)), "tc_build_tree")
 
+    def validate(self, stream=None):
+        """Checks the last build's invariants (tc_tree_validate); returns (ok, message)."""
+        buf = C.create_string_buffer(512)
+        st = lib().tc_tree_validate(self.ctx.handle, C.byref(self.t), buf, len(buf), _stream(stream))
+        if st not in (0, -10):
+            check(st, "tc_tree_validate")
+        return st == 0, buf.value.decode()
+
+    def export(self, path: str, stream=None):
+        """Writes the last build to a tc-tree file (tc_tree_export); read it with tc.load_tree."""
+        check(lib().tc_tree_export(self.ctx.handle, C.byref(self.t), str(path).encode(), _stream(stream)),
+              "tc_tree_export")
+
     def read_counts(self, stream=None):
         check(lib().tc_tree_read_counts(self.ctx.handle, C.byref(self.t), _stream(stream)), "tc_tree_read_counts")
         return dict(n_points=self.t.n_points, n_unique=self.t.n_unique, n_brt=self.t.n_brt, n_nodes=self.t.n_nodes)

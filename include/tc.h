@@ -48,7 +48,9 @@ enum tc_status {
     TC_ERR_CAPACITY = -6,    /* output capacity too small */
     TC_ERR_BUSY = -7,        /* queue full / try again */
     TC_ERR_NOT_FOUND = -8,   /* unknown request id */
-    TC_ERR_NO_DEVICE = -9    /* no usable GPU */
+    TC_ERR_NO_DEVICE = -9,   /* no usable GPU */
+    TC_ERR_BAD_TREE = -10,   /* tc_tree_validate*: an invariant does not hold */
+    TC_ERR_IO = -11          /* file could not be written */
 };
 
 #define TC_MAX_SORT_N (1u << 30)
@@ -284,6 +286,9 @@ typedef struct {
     uint32_t* point_index;    /* [capacity] */
     uint32_t* index_alt;      /* [capacity] sort ping-pong space for the index */
     uint32_t* leaf_first;     /* [capacity+1] */
+    /* the last build's parameters (set by tc_build_tree) */
+    float min_coord, range;
+    int link_leaves;
 } tc_tree;
 
 #define TC_TREE_POINT_INDEX 1 /* tc_tree_alloc_ex flag: also build the point -> leaf map */
@@ -295,6 +300,53 @@ int tc_build_tree(tc_ctx* ctx, const float* pts_xyzw, uint32_t n, float min_coor
                   int link_leaves, tc_tree* tree, tc_stream stream);
 /* Synchronise the stream and copy the device counts into the tree struct. */
 int tc_tree_read_counts(tc_ctx* ctx, tc_tree* tree, tc_stream stream);
+
+/* ---- Tree export and validation (host side).  No reference counterpart
+ *      beyond the commented-out little-endian dump of the sorted keys at
+ *      src/main.rs:761-768.
+ *
+ * tc_tree_host describes one tree in host memory (the arrays of tc_tree after
+ * a download; optional arrays may be NULL).  Node and radix arrays have the
+ * layouts documented above. */
+typedef struct {
+    int code_bits;               /* 30 or 63 */
+    float min_coord, range;
+    int link_leaves;             /* leaves were linked into the octree */
+    uint32_t n_points, n_unique, n_nodes;
+    const void* sorted;          /* [n_points] sorted codes (optional) */
+    const void* unique;          /* [n_unique] */
+    const uint8_t* prefix_n;     /* [n_unique-1] each */
+    const uint8_t* has_leaf_left;
+    const uint8_t* has_leaf_right;
+    const int32_t* left_child;
+    const int32_t* parent;
+    const uint32_t* edge_count;  /* [n_unique-1] */
+    const uint32_t* offsets;     /* [n_unique] */
+    const tc_oct_node* nodes;    /* [n_nodes] */
+    const uint32_t* point_index; /* [n_points] (optional) */
+    const uint32_t* leaf_first;  /* [n_unique+1] (optional) */
+} tc_tree_host;
+
+/* Checks every invariant that defines the tree, independently of how it was
+ * built: unique codes strictly increasing; radix tree links (children, parents,
+ * each leaf and non-root node referenced once), each node's key range with the
+ * node index at one end, prefix_n = common prefix of the range and the split
+ * at its first differing bit; edge counts and offsets; octree child links,
+ * halving cell sizes, corners equal to the decoded cell prefixes, every node
+ * reachable once from the root, and (link_leaves) every unique code found by
+ * descending from the root into its leaf slot; the point -> leaf map when
+ * present.  Returns TC_OK or TC_ERR_BAD_TREE with the first violation
+ * written to msg (NUL-terminated, at most msg_len bytes; msg may be NULL). */
+int tc_tree_validate_host(const tc_tree_host* tree, char* msg, size_t msg_len);
+/* Writes the tree to a file: 8-byte magic "TCTREE\0\1", a u64 header
+ * length, a JSON header (sizes, parameters and for each array its name,
+ * little-endian dtype, count and byte offset), then the arrays, each 64-byte
+ * aligned.  Returns TC_OK or TC_ERR_IO. */
+int tc_tree_write_host(const tc_tree_host* tree, const char* path);
+/* The same on a device tree of the last tc_build_tree (synchronises the
+ * stream and copies the arrays to host memory). */
+int tc_tree_validate(tc_ctx* ctx, tc_tree* tree, char* msg, size_t msg_len, tc_stream stream);
+int tc_tree_export(tc_ctx* ctx, tc_tree* tree, const char* path, tc_stream stream);
 
 /* ---- Batched builds (no reference counterpart: the reference builds one cloud
  *      per call, src/main.rs:60-170).  Many clouds of uneven sizes, concatenated
