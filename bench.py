@@ -195,7 +195,7 @@ def bench_sort_pairs_u64(ctx, log2n, steps):
     }
 
 
-def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04):
+def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04, plain=True):
     import numpy as np
     import torch
 
@@ -231,6 +231,10 @@ def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04):
         if i >= 2:
             lat.append(time.perf_counter() - t0)
     batched_s = float(np.mean(lat))
+    if not plain:
+        batch.close()
+        return {"total_points": total, "batched_ms": round(batched_s * 1e3, 3),
+                "batched_Mpoints_per_s": round(total / batched_s / 1e6, 1)}
 
     # Plain: one tc_build_tree per cloud, one cloud at a time.
     tree = tc.Tree(ctx, int(sizes.max()), 30)
@@ -335,6 +339,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true", help="fixed sort launch shape instead of the tuned ones")
     ap.add_argument("--retune", action="store_true", help="run the autotuner instead of loading its saved table")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched workload's graph replays (for rocprofv3 kernel statistics)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only build the headline tree 3 times, then run reference kernels of known traffic "
                          "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
@@ -359,6 +365,9 @@ def main():
         tuning = "loaded " + os.path.relpath(TUNING_FILE, ROOT)
     if args.traffic_probe:
         traffic_probe(ctx, pts_h)
+        return
+    if args.batched_only:
+        print(json.dumps({"batched": bench_batched(ctx, 1024, args.steps, plain=False)}), flush=True)
         return
     pts = torch.from_numpy(pts_h).cuda()
     tree = tc.Tree(ctx, n, 30)
