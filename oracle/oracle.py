@@ -70,6 +70,9 @@ def lib():
             "tco_build_octree_u32": (u32, [P, P, P, P, P, P, P, P, P, u32, f32, f32, i32]),
             "tco_build_octree_u64": (u32, [P, P, P, P, P, P, P, P, P, u32, f32, f32, i32]),
             "tco_octree_node_bound": (u64, [u64, i32]),
+            "tco_mt_build_tree_u32": (u32, [P, u32, f32, f32, i32] + [P] * 13 + [u64, i32, P]),
+            "tco_mt_build_tree_u64": (u32, [P, u32, f32, f32, i32] + [P] * 13 + [u64, i32, P]),
+            "tco_std_sort_ms": (C.c_double, [P, u32, P]),
         }
         for name, (res, argt) in sig.items():
             fn = getattr(L, name)
@@ -242,3 +245,44 @@ def build_tree(points: np.ndarray, min_coord: float, range_: float, code_bits: i
     nodes = build_octree(offsets, ec, uniq, tree, min_coord, range_, link_leaves)
     return dict(codes=codes, sorted=sorted_codes, unique=uniq, edge_count=ec, offsets=offsets, nodes=nodes, **tree,
                 **extra)
+
+
+# ------------------------------------------------- multi-threaded CPU baseline
+MT_STAGES = ("morton", "sort", "unique", "radix_tree", "edge_count", "scan", "octree")
+
+
+def build_tree_mt(points: np.ndarray, min_coord: float, range_: float, code_bits: int = 30, threads: int = 0,
+                  link_leaves=True, timings: dict | None = None):
+    """The whole pipeline on `threads` host threads (cpu_mt.cpp; 0 = all
+    cores): same outputs as build_tree.  `timings` receives per-stage ms."""
+    pts = _pts(points)
+    n = len(pts)
+    threads = threads or (os.cpu_count() or 1)
+    kdt = np.uint32 if code_bits == 30 else np.uint64
+    codes, srt, tmp, uniq = (np.empty(max(n, 1), kdt) for _ in range(4))
+    nu = C.c_uint32(0)
+    nb = max(n - 1, 1)
+    pn, hll, hlr = (np.zeros(nb, np.uint8) for _ in range(3))
+    left, par = np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+    ec, offs = np.zeros(nb, np.uint32), np.zeros(nb + 1, np.uint32)
+    cap = int(lib().tco_octree_node_bound(max(n, 1), code_bits))
+    nodes = np.zeros(max(cap, 1), OCT_NODE_DTYPE)
+    st = np.zeros(7, np.float64)
+    fn = lib().tco_mt_build_tree_u32 if code_bits == 30 else lib().tco_mt_build_tree_u64
+    total = fn(_p(pts), n, min_coord, range_, threads, _p(codes), _p(srt), _p(tmp), _p(uniq), C.byref(nu), _p(pn),
+               _p(hll), _p(hlr), _p(left), _p(par), _p(ec), _p(offs), _p(nodes), cap, int(bool(link_leaves)), _p(st))
+    if total == 0xFFFFFFFF:
+        raise RuntimeError("octree node bound exceeded")
+    u = nu.value
+    b = max(u - 1, 0)
+    if timings is not None:
+        timings.update(dict(zip(MT_STAGES, st.tolist())))
+    return dict(codes=codes[:n], sorted=srt[:n], unique=uniq[:u], prefix_n=pn[:b], has_leaf_left=hll[:b],
+                has_leaf_right=hlr[:b], left_child=left[:b], parent=par[:b], edge_count=ec[:b],
+                offsets=offs[:b + 1] if b else np.zeros(1, np.uint32), nodes=nodes[:total].copy())
+
+
+def std_sort_ms(codes: np.ndarray) -> float:
+    c = np.ascontiguousarray(codes, dtype=np.uint32)
+    work = np.empty_like(c)
+    return float(lib().tco_std_sort_ms(_p(c), len(c), _p(work)))

@@ -303,7 +303,7 @@ static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
  * n_keys codes (Q7); parent[0] = 0, the value the reference's zero-initialised
  * host array keeps (src/main.rs:103). */
 #define TCO_BRT_BODY(DELTA)                                                         \
-    for (int64_t i = 0; i < nb; ++i) {                                              \
+    for (int64_t i = i0; i < i1; ++i) {                                             \
         int d = (DELTA(c, nk, i, i + 1) - DELTA(c, nk, i, i - 1)) >= 0 ? 1 : -1;    \
         int dmin = DELTA(c, nk, i, i - d);                                          \
         int64_t lmax = 2;                                                           \
@@ -327,20 +327,36 @@ static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
         if (!hlr[i]) parent[gamma + 1] = (int32_t)i;                                \
     }
 
-void tco_build_radix_tree_u32(const uint32_t* c, uint32_t n_keys, uint8_t* prefix_n, uint8_t* hll,
-                              uint8_t* hlr, int32_t* left_child, int32_t* parent) {
-    int64_t nk = n_keys, nb = nk > 0 ? nk - 1 : 0;
-    if (nb > 0) parent[0] = 0;
+/* Nodes [i0, i1) only (every node is independent; each parent[] entry has
+ * one writer), so index ranges can be built on separate threads. */
+void tco_build_radix_tree_range_u32(const uint32_t* c, uint32_t n_keys, uint32_t i0, uint32_t i1,
+                                    uint8_t* prefix_n, uint8_t* hll, uint8_t* hlr, int32_t* left_child,
+                                    int32_t* parent) {
+    int64_t nk = n_keys;
     /* d is never 0 for strictly increasing codes: the neighbours' prefix
      * lengths differ (Karras 2012, sec. 4).  Ties (duplicate codes) take +1. */
     TCO_BRT_BODY(delta30)
 }
 
+void tco_build_radix_tree_range_u64(const uint64_t* c, uint32_t n_keys, uint32_t i0, uint32_t i1,
+                                    uint8_t* prefix_n, uint8_t* hll, uint8_t* hlr, int32_t* left_child,
+                                    int32_t* parent) {
+    int64_t nk = n_keys;
+    TCO_BRT_BODY(delta63)
+}
+
+void tco_build_radix_tree_u32(const uint32_t* c, uint32_t n_keys, uint8_t* prefix_n, uint8_t* hll,
+                              uint8_t* hlr, int32_t* left_child, int32_t* parent) {
+    uint32_t nb = n_keys > 0 ? n_keys - 1 : 0;
+    if (nb > 0) parent[0] = 0;
+    tco_build_radix_tree_range_u32(c, n_keys, 0, nb, prefix_n, hll, hlr, left_child, parent);
+}
+
 void tco_build_radix_tree_u64(const uint64_t* c, uint32_t n_keys, uint8_t* prefix_n, uint8_t* hll,
                               uint8_t* hlr, int32_t* left_child, int32_t* parent) {
-    int64_t nk = n_keys, nb = nk > 0 ? nk - 1 : 0;
+    uint32_t nb = n_keys > 0 ? n_keys - 1 : 0;
     if (nb > 0) parent[0] = 0;
-    TCO_BRT_BODY(delta63)
+    tco_build_radix_tree_range_u64(c, n_keys, 0, nb, prefix_n, hll, hlr, left_child, parent);
 }
 
 /* -------------------------------------------------------------- Edge count */
@@ -371,17 +387,19 @@ uint32_t tco_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n) {
 
 /* ---------------------------------------------------------------- Octree */
 
-/* SetChild, shader/octree.comp:22-25. */
+/* SetChild, shader/octree.comp:22-25.  The mask is or-ed atomically (Q15):
+ * ranges of radix nodes may be emitted on separate threads, and several of
+ * them set bits of one parent's mask; each children[] slot has one writer. */
 static void set_child(tco_oct_node* node, int which, int32_t idx) {
     node->children[which] = idx;
-    node->child_node_mask |= 1 << which;
+    __atomic_fetch_or(&node->child_node_mask, 1 << which, __ATOMIC_RELAXED);
 }
 
 /* SetLeaf, shader/octree.comp:27-30, reproduced literally (Q16): the leaf
  * mask is cleared, so with zero-initialised nodes it stays 0. */
 static void set_leaf(tco_oct_node* node, int which, int32_t idx) {
     node->children[which] = idx;
-    node->child_leaf_mask &= ~(1 << which);
+    __atomic_fetch_and(&node->child_leaf_mask, ~(1 << which), __ATOMIC_RELAXED);
 }
 
 /* Nearest radix-tree ancestor-or-self with a non-zero edge count.  With the
@@ -402,11 +420,8 @@ static uint32_t contributing(uint32_t i, const uint32_t* ec, const int32_t* pare
  *  - child slots are set with an atomic-or mask (Q15), written here sequentially;
  *  - leaves are linked when link_leaves != 0 (the reference's disabled pass). */
 #define TCO_OCT_BODY(BITS, DECODE)                                                            \
-    if (n_brt == 0) return 0;                                                                 \
-    uint32_t total = offsets[n_brt];                                                          \
-    memset(nodes, 0, (size_t)total * sizeof(tco_oct_node));                                   \
     int root_level = prefix_n[0] / 3;                                                         \
-    for (uint32_t i = 0; i < n_brt; ++i) {                                                    \
+    for (uint32_t i = i0; i < i1; ++i) {                                                      \
         uint32_t cnt = ec[i];                                                                 \
         if (cnt == 0) continue;                                                               \
         int bottom = prefix_n[i] / 3;                                                         \
@@ -429,7 +444,7 @@ static uint32_t contributing(uint32_t i, const uint32_t* ec, const int32_t* pare
         }                                                                                     \
     }                                                                                         \
     if (link_leaves) {                                                                        \
-        for (uint32_t i = 0; i < n_brt; ++i) {                                                \
+        for (uint32_t i = i0; i < i1; ++i) {                                                  \
             int leaf_level = prefix_n[i] / 3 + 1;                                             \
             for (int side = 0; side < 2; ++side) {                                            \
                 if (!(side == 0 ? hll[i] : hlr[i])) continue;                                 \
@@ -439,21 +454,46 @@ static uint32_t contributing(uint32_t i, const uint32_t* ec, const int32_t* pare
                 set_leaf(&nodes[offsets[a]], which, leaf);                                    \
             }                                                                                 \
         }                                                                                     \
-    }                                                                                         \
-    return total;
+    }
+
+/* Radix nodes [i0, i1) only, into zeroed nodes (the result does not depend on
+ * how the range is split). */
+void tco_build_octree_range_u32(tco_oct_node* nodes, const uint32_t* offsets, const uint32_t* ec,
+                                const uint32_t* codes, const uint8_t* prefix_n, const int32_t* parent,
+                                const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
+                                uint32_t i0, uint32_t i1, float min_coord, float range, int link_leaves) {
+    TCO_OCT_BODY(30, tco_morton30_decode)
+}
+
+void tco_build_octree_range_u64(tco_oct_node* nodes, const uint32_t* offsets, const uint32_t* ec,
+                                const uint64_t* codes, const uint8_t* prefix_n, const int32_t* parent,
+                                const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
+                                uint32_t i0, uint32_t i1, float min_coord, float range, int link_leaves) {
+    TCO_OCT_BODY(63, tco_morton63_decode)
+}
 
 uint32_t tco_build_octree_u32(tco_oct_node* nodes, const uint32_t* offsets, const uint32_t* ec,
                               const uint32_t* codes, const uint8_t* prefix_n, const int32_t* parent,
                               const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                               uint32_t n_brt, float min_coord, float range, int link_leaves) {
-    TCO_OCT_BODY(30, tco_morton30_decode)
+    if (n_brt == 0) return 0;
+    uint32_t total = offsets[n_brt];
+    memset(nodes, 0, (size_t)total * sizeof(tco_oct_node));
+    tco_build_octree_range_u32(nodes, offsets, ec, codes, prefix_n, parent, left_child, hll, hlr, 0, n_brt,
+                               min_coord, range, link_leaves);
+    return total;
 }
 
 uint32_t tco_build_octree_u64(tco_oct_node* nodes, const uint32_t* offsets, const uint32_t* ec,
                               const uint64_t* codes, const uint8_t* prefix_n, const int32_t* parent,
                               const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                               uint32_t n_brt, float min_coord, float range, int link_leaves) {
-    TCO_OCT_BODY(63, tco_morton63_decode)
+    if (n_brt == 0) return 0;
+    uint32_t total = offsets[n_brt];
+    memset(nodes, 0, (size_t)total * sizeof(tco_oct_node));
+    tco_build_octree_range_u64(nodes, offsets, ec, codes, prefix_n, parent, left_child, hll, hlr, 0, n_brt,
+                               min_coord, range, link_leaves);
+    return total;
 }
 
 uint64_t tco_octree_node_bound(uint64_t n_keys, int code_bits) {

@@ -72,6 +72,14 @@ void tco_build_radix_tree_u64(const uint64_t* codes, uint32_t n_keys, uint8_t* p
                               uint8_t* has_leaf_left, uint8_t* has_leaf_right,
                               int32_t* left_child, int32_t* parent);
 
+/* Nodes [i0, i1) only (parent[0] is not written); ranges may run on separate threads. */
+void tco_build_radix_tree_range_u32(const uint32_t* codes, uint32_t n_keys, uint32_t i0, uint32_t i1,
+                                    uint8_t* prefix_n, uint8_t* has_leaf_left, uint8_t* has_leaf_right,
+                                    int32_t* left_child, int32_t* parent);
+void tco_build_radix_tree_range_u64(const uint64_t* codes, uint32_t n_keys, uint32_t i0, uint32_t i1,
+                                    uint8_t* prefix_n, uint8_t* has_leaf_left, uint8_t* has_leaf_right,
+                                    int32_t* left_child, int32_t* parent);
+
 /* ---- Edge count (shader/edge_count.comp:30-41) with the root slot = 1. */
 void tco_edge_count(const uint8_t* prefix_n, const int32_t* parent, uint32_t* edge_count,
                     uint32_t n_brt, int code_bits);
@@ -91,6 +99,38 @@ uint32_t tco_build_octree_u64(tco_oct_node* nodes, const uint32_t* offsets, cons
                               const int32_t* left_child, const uint8_t* has_leaf_left,
                               const uint8_t* has_leaf_right, uint32_t n_brt, float min_coord,
                               float range, int link_leaves);
+
+/* Radix nodes [i0, i1) only, into nodes zeroed beforehand (offsets[n_brt] entries). */
+void tco_build_octree_range_u32(tco_oct_node* nodes, const uint32_t* offsets, const uint32_t* edge_count,
+                                const uint32_t* codes, const uint8_t* prefix_n, const int32_t* parent,
+                                const int32_t* left_child, const uint8_t* has_leaf_left,
+                                const uint8_t* has_leaf_right, uint32_t i0, uint32_t i1, float min_coord,
+                                float range, int link_leaves);
+void tco_build_octree_range_u64(tco_oct_node* nodes, const uint32_t* offsets, const uint32_t* edge_count,
+                                const uint64_t* codes, const uint8_t* prefix_n, const int32_t* parent,
+                                const int32_t* left_child, const uint8_t* has_leaf_left,
+                                const uint8_t* has_leaf_right, uint32_t i0, uint32_t i1, float min_coord,
+                                float range, int link_leaves);
+
+/* ---- Multi-threaded CPU build of the whole pipeline (cpu_mt.cpp): the CPU
+ * baseline.  Same outputs as the functions above; nodes holds node_cap
+ * entries; returns the node count (0xFFFFFFFF when above node_cap) and, in
+ * stage_ms[7] (optional), the time of morton, sort, unique, radix tree, edge
+ * count, scan, octree. */
+uint32_t tco_mt_build_tree_u32(const float* pts_xyzw, uint32_t n, float min_coord, float range, int threads,
+                               uint32_t* codes, uint32_t* sorted, uint32_t* tmp, uint32_t* unique,
+                               uint32_t* n_unique, uint8_t* prefix_n, uint8_t* has_leaf_left,
+                               uint8_t* has_leaf_right, int32_t* left_child, int32_t* parent,
+                               uint32_t* edge_count, uint32_t* offsets, tco_oct_node* nodes, uint64_t node_cap,
+                               int link_leaves, double* stage_ms);
+uint32_t tco_mt_build_tree_u64(const float* pts_xyzw, uint32_t n, float min_coord, float range, int threads,
+                               uint64_t* codes, uint64_t* sorted, uint64_t* tmp, uint64_t* unique,
+                               uint32_t* n_unique, uint8_t* prefix_n, uint8_t* has_leaf_left,
+                               uint8_t* has_leaf_right, int32_t* left_child, int32_t* parent,
+                               uint32_t* edge_count, uint32_t* offsets, tco_oct_node* nodes, uint64_t node_cap,
+                               int link_leaves, double* stage_ms);
+/* std::sort of a copy of n codes into work; returns its milliseconds. */
+double tco_std_sort_ms(const uint32_t* codes, uint32_t n, uint32_t* work);
 
 /* Upper bound on the octree node count for n_keys unique codes of code_bits
  * bits (1 + sum over levels of min(8^L, n_keys/2)). */

@@ -476,3 +476,25 @@ def test_oracle_reproduces_golden_pairs(oracle):
     assert np.array_equal(sk, g["sorted_keys"]) and np.array_equal(sv, g["sorted_vals"])
     order = np.argsort(g["keys"], kind="stable")
     assert np.array_equal(sv, g["vals"][order])
+
+
+# ------------------------------------------------ multi-threaded CPU baseline
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 70_000])
+def test_mt_baseline_matches_oracle(oracle, tclib, bits, threads, n):
+    """The multi-threaded CPU baseline builds exactly the oracle's tree."""
+    pts = tclib.gen_gaussian_mixture(n, 50 + n, 16, 2.0)
+    want = oracle.build_tree(pts, 0.0, 1024.0, bits)
+    t = {}
+    got = oracle.build_tree_mt(pts, 0.0, 1024.0, bits, threads=threads, timings=t)
+    for k in ("codes", "sorted", "unique", "prefix_n", "has_leaf_left", "has_leaf_right", "left_child", "parent",
+              "edge_count", "offsets"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(got["nodes"].view(np.uint8), want["nodes"].view(np.uint8))
+    assert set(t) == set(oracle.MT_STAGES)
+
+
+def test_std_sort_baseline(oracle):
+    codes = rng(3).integers(0, 1 << 30, 100_000, dtype=np.uint64).astype(np.uint32)
+    assert oracle.std_sort_ms(codes) >= 0.0
