@@ -68,6 +68,7 @@ struct tc_batch {
 namespace {
 
 constexpr int SEG_BLOCK = 256;
+constexpr int SEG_ITEMS = 8;  // point loads in flight per lane in the Morton kernel
 constexpr int SEG_BRT_TILE = 1024;  // slots per workgroup step of the tree kernels
 constexpr int SEG_BRT_HALO = 512;   // LDS window halo of the octree walk
 constexpr int SEG_SEARCH_HALO = 128;     // LDS window halo of the radix-tree searches
@@ -174,14 +175,25 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
         using gptr = const __attribute__((address_space(1))) v4*;
         const gptr src = (p.desc[3] & PLAN_IN_PTRS) ? (gptr)reinterpret_cast<const v4*>(p.in_ptr[c])
                                                     : (gptr)reinterpret_cast<const v4*>(pts + c0);
-        for (uint32_t i = start + tid; i < end; i += SEG_BLOCK) {
-            const v4 qv = src[i - c0];
-            const float4 q = make_float4(qv.x, qv.y, qv.z, qv.w);
-            K code;
-            if constexpr (sizeof(K) == 4) code = d_encode30(q.x, q.y, q.z, mn, rg);
-            else code = d_encode63(q.x, q.y, q.z, mn, rg);
-            codes[i] = code;
-            lds_count_digits<K, DIG>(s_h, code);
+        // SEG_ITEMS point loads in flight per lane before any is encoded.
+        for (uint32_t i0 = start + tid; i0 < end; i0 += SEG_BLOCK * SEG_ITEMS) {
+            v4 qv[SEG_ITEMS];
+#pragma unroll
+            for (int u = 0; u < SEG_ITEMS; ++u) {
+                const uint32_t i = i0 + u * SEG_BLOCK;
+                if (i < end) qv[u] = src[i - c0];
+            }
+#pragma unroll
+            for (int u = 0; u < SEG_ITEMS; ++u) {
+                const uint32_t i = i0 + u * SEG_BLOCK;
+                if (i < end) {
+                    K code;
+                    if constexpr (sizeof(K) == 4) code = d_encode30(qv[u].x, qv[u].y, qv[u].z, mn, rg);
+                    else code = d_encode63(qv[u].x, qv[u].y, qv[u].z, mn, rg);
+                    codes[i] = code;
+                    lds_count_digits<K, DIG>(s_h, code);
+                }
+            }
         }
         __syncthreads();
     }
