@@ -355,6 +355,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0xC0FFEE02)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline build only")
+    ap.add_argument("--markers", action="store_true",
+                    help="roctx ranges around each launch and build (for rocprofv3 --marker-trace)")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true", help="fixed sort launch shape instead of the tuned ones")
     ap.add_argument("--retune", action="store_true", help="run the autotuner instead of loading its saved table")
@@ -376,6 +378,8 @@ def main():
     n = 1 << args.log2n
     pts_h = tc.gen_gaussian_mixture(n, args.seed, 64, 8.0, 1024.0)
     ctx = tc.Context(0)
+    if args.markers:
+        ctx.markers(True)
     tuning = None
     if args.retune:
         tuning = tc.tune.autotune(ctx, n)

@@ -639,6 +639,7 @@ extern "C" int tc_batch_set(tc_batch* b, uint32_t n_clouds, const uint32_t* coun
 extern "C" int tc_batch_build(tc_batch* b, const float* pts, int link_leaves, tc_stream stream) {
     if (!b || (b->n_points && !pts && !b->in_ptrs)) return TC_ERR_INVALID;
     if (reinterpret_cast<uintptr_t>(pts) & 15u) return TC_ERR_INVALID;
+    const MarkRange mark(b->ctx, "tc_batch_build");
     return run_pipeline(b, pts, link_leaves, as_stream(stream));
 }
 
@@ -676,6 +677,7 @@ extern "C" int tc_batch_capture(tc_batch* b, const float* pts, int link_leaves, 
 extern "C" int tc_batch_replay(tc_batch* b, tc_stream stream) {
     if (!b || !b->exec) return TC_ERR_INVALID;
     if (!b->graph_pts && !b->in_ptrs && b->n_points) return TC_ERR_INVALID;  // captured without an input array
+    const MarkRange mark(b->ctx, "tc_batch_replay");
     ProfScope prof(b->ctx, as_stream(stream), K_SEG);
     TC_HIP(hipGraphLaunch(b->exec, as_stream(stream)));
     return TC_OK;

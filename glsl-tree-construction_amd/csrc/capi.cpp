@@ -8,6 +8,8 @@
 // mirrors copy in, run, copy out.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -101,11 +103,36 @@ hipEvent_t tc_prof_event(tc_ctx* ctx) {
     return e;
 }
 
-extern "C" const char* tc_kernel_name(int id) {
+const char* tc_kernel_name_internal(int id) {
     static const char* names[K_COUNT] = {"morton", "histogram", "binning", "unique", "scan", "radix_tree",
                                          "edge_count", "edge_count_scan", "octree", "octree_link",
                                          "fill", "decode", "segmented"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "?";
+}
+
+extern "C" const char* tc_kernel_name(int id) { return tc_kernel_name_internal(id); }
+
+// ------------------------------------------------------------ roctx markers
+TcRoctx g_tc_roctx;
+
+extern "C" int tc_set_markers(tc_ctx* ctx, int on) {
+    if (!ctx) return TC_ERR_INVALID;
+    if (on && !g_tc_roctx.push) {
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lock(mu);
+        if (!g_tc_roctx.push) {
+            void* h = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_LOCAL);
+            if (!h) h = dlopen("libroctx64.so.4", RTLD_NOW | RTLD_LOCAL);
+            if (!h) return TC_ERR_NOT_FOUND;
+            auto push = reinterpret_cast<int (*)(const char*)>(dlsym(h, "roctxRangePushA"));
+            auto pop = reinterpret_cast<int (*)()>(dlsym(h, "roctxRangePop"));
+            if (!push || !pop) return TC_ERR_NOT_FOUND;
+            g_tc_roctx.pop = pop;
+            g_tc_roctx.push = push;
+        }
+    }
+    ctx->markers_on = on != 0;
+    return TC_OK;
 }
 
 extern "C" int tc_profile_enable(tc_ctx* ctx, int on) {
@@ -333,6 +360,7 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     if (!ctx || !t || (n && !pts)) return TC_ERR_INVALID;
     if (n > t->capacity) return TC_ERR_CAPACITY;
     hipStream_t s = as_stream(stream);
+    const MarkRange mark(ctx, "tc_build_tree");
     t->n_points = n;
     t->min_coord = min_coord;
     t->range = range;

@@ -54,6 +54,15 @@ enum TcKernelId {
     K_COUNT
 };
 
+// roctx marker entry points, resolved from the ROCm marker library when a
+// context first enables markers (tc_set_markers); null until then.
+struct TcRoctx {
+    int (*push)(const char*) = nullptr;
+    int (*pop)() = nullptr;
+};
+extern TcRoctx g_tc_roctx;
+const char* tc_kernel_name_internal(int id);
+
 struct ProfRec {
     int id;
     hipEvent_t a, b;
@@ -64,6 +73,7 @@ struct tc_ctx {
     // Event profiler: when on, every launch is bracketed by two events on its
     // stream; tc_profile_collect folds the elapsed times into per-kernel sums.
     bool prof_on = false;
+    bool markers_on = false;  // roctx ranges around calls and launches
     std::vector<ProfRec> prof_pending;
     std::vector<hipEvent_t> prof_pool;
     double prof_ms[K_COUNT] = {};
@@ -129,13 +139,16 @@ struct ProfScope {
     hipStream_t s;
     int id;
     hipEvent_t a = nullptr;
+    bool marked = false;
     ProfScope(tc_ctx* c, hipStream_t st, int kid) : ctx(c), s(st), id(kid) {
+        if (ctx->markers_on && g_tc_roctx.push) marked = g_tc_roctx.push(tc_kernel_name_internal(id)) >= 0;
         if (ctx->prof_on) {
             a = tc_prof_event(ctx);
             if (a) (void)hipEventRecord(a, s);
         }
     }
     ~ProfScope() {
+        if (marked) g_tc_roctx.pop();
         if (a) {
             hipEvent_t b = tc_prof_event(ctx);
             if (b) {
@@ -143,6 +156,17 @@ struct ProfScope {
                 ctx->prof_pending.push_back({id, a, b});
             }
         }
+    }
+};
+
+// A roctx range around a whole call (host-side: the time the call spends
+// enqueuing its launches); no-op unless the context has markers on.
+struct MarkRange {
+    bool on;
+    MarkRange(const tc_ctx* ctx, const char* name)
+        : on(ctx && ctx->markers_on && g_tc_roctx.push && g_tc_roctx.push(name) >= 0) {}
+    ~MarkRange() {
+        if (on) g_tc_roctx.pop();
     }
 };
 

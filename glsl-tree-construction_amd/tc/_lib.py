@@ -216,6 +216,7 @@ SIGNATURES = {
     "tc_set_sort_params": (I, [P, I, I]),
     "tc_set_lookback_spin_limit": (I, [P, U32]),
     "tc_profile_enable": (I, [P, I]),
+    "tc_set_markers": (I, [P, I]),
     "tc_profile_collect": (I, [P]),
     "tc_profile_reset": (I, [P]),
     "tc_profile_get": (I, [P, I, C.POINTER(C.c_double), C.POINTER(U64)]),

@@ -94,6 +94,10 @@ class Context:
                 out[L.tc_kernel_name(kid).decode()] = (ms.value, cnt.value)
         return out
 
+    def markers(self, on: bool = True):
+        """roctx ranges around every launch and build call (tc_set_markers)."""
+        check(lib().tc_set_markers(self._h, int(bool(on))), "tc_set_markers")
+
     def sort_params(self):
         b, k = C.c_int(), C.c_int()
         check(lib().tc_get_sort_params(self._h, C.byref(b), C.byref(k)), "tc_get_sort_params")

@@ -105,6 +105,13 @@ int tc_set_lookback_spin_limit(tc_ctx* ctx, uint32_t polls);
  * 5 radix_tree, 6 edge_count, 7 edge_count_scan, 8 octree,
  * 9 octree_link, 10 fill, 11 decode, 12 segmented. */
 int tc_profile_enable(tc_ctx* ctx, int on);
+/* roctx ranges (no reference counterpart; its timers are Instant::now() around
+ * each submit, e.g. src/morton.rs:327-352): with markers on, every kernel
+ * launch and every tc_build_tree / tc_batch_build / tc_batch_replay call is a
+ * named range for `rocprofv3 --marker-trace` (host-side: the time spent
+ * enqueuing).  The ROCm marker library is loaded on first use;
+ * TC_ERR_NOT_FOUND if it is absent. */
+int tc_set_markers(tc_ctx* ctx, int on);
 int tc_profile_collect(tc_ctx* ctx);
 int tc_profile_reset(tc_ctx* ctx);
 int tc_profile_get(tc_ctx* ctx, int kernel_id, double* total_ms, uint64_t* launches);

@@ -316,3 +316,49 @@ def test_lookback_fallback_path(gpu_ctx, oracle, tclib):
         assert np.array_equal(host(k, np.uint64), sk) and np.array_equal(host(v, np.uint32), sv)
     finally:
         assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 1 << 15) == 0
+
+
+def test_reference_sort_input_10M_descending(gpu_ctx, oracle):
+    """The reference's own sort input, keys 10^7 ... 1 (src/main.rs:189-190):
+    global histogram against the oracle and its closed form (digit 0, bin 1 =
+    39,062; bin 255 = 9,960,938; digit 3, bins >= 1 = 10^7), each binning pass
+    against the oracle's, and the full sort against 1 ... 10^7."""
+    n = 10_000_000
+    keys = (n - np.arange(n, dtype=np.int64)).astype(np.uint32)
+    hist = empty(1024, np.uint32)
+    gpu_ctx.global_histogram(dev(keys), hist)
+    h = host(hist, np.uint32)
+    assert np.array_equal(h, oracle.histogram(keys))
+    assert h[1] == 39_062 and h[255] == 9_960_938 and np.all(h[768 + 1:] == n)
+    cur = keys
+    for p in range(4):
+        dst = empty(n, np.uint32)
+        gpu_ctx.binning_pass(dev(cur), dst, None, None, dev(h), 8 * p)
+        want, _ = oracle.binning_pass(cur, None, h[256 * p:256 * p + 256], 8 * p)
+        got = host(dst, np.uint32, n)
+        assert np.array_equal(got, want), p
+        cur = got
+    k = dev(keys)
+    gpu_ctx.sort(k, empty(n, np.uint32))
+    assert np.array_equal(host(k, np.uint32, n), np.arange(1, n + 1, dtype=np.uint32))
+
+
+def test_markers_do_not_change_results(gpu_ctx, oracle, tclib):
+    """roctx ranges on (the ROCm marker library is part of the image): the
+    build is unchanged."""
+    import tc
+
+    pts = tclib.gen_gaussian_mixture(60_000, 77, 8, 3.0)
+    gpu_ctx.markers(True)
+    try:
+        tree = tc.Tree(gpu_ctx, len(pts), 30)
+        tree.build(dev(pts), len(pts), 0.0, 1024.0)
+        got = tree.download()
+        tree.free()
+    finally:
+        gpu_ctx.markers(False)
+    want = oracle.build_tree(pts, 0.0, 1024.0, 30)
+    for k in STAGE_KEYS:
+        if k in want and k in got:
+            assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(got["nodes"].view(np.uint8), want["nodes"].view(np.uint8))
