@@ -223,12 +223,12 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_prefix(Plan p, uint32_t* __re
 
 // One OneSweep pass over every cloud: tiles claimed in order, look-back
 // confined to the cloud's own tiles.
-template <typename K, int BLOCK, int KPT>
+template <typename K, int BLOCK, int KPT, int RB>
 __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(Plan p, const K* __restrict__ sk, K* __restrict__ dk,
                                                         uint32_t shift, uint32_t pass, const uint32_t* __restrict__ seg_hist,
                                                         uint32_t* __restrict__ lb_cur, uint32_t* __restrict__ lb_next,
                                                         uint32_t spin_limit) {
-    using Sh = OnesweepShared<K, BLOCK, KPT>;
+    using Sh = OnesweepShared<K, BLOCK, KPT, RB>;
     constexpr int DIG = (int)sizeof(K);
     __shared__ Sh sm;
     const uint32_t C = p.desc[0], T = p.desc[1];
@@ -243,9 +243,9 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(Plan p, const K* __restr
     const uint32_t base = seg_base + (tile - first) * Sh::TILE;
     const uint32_t end = p.pt_off[c + 1];
     const uint32_t cnt = end - base < Sh::TILE ? end - base : Sh::TILE;
-    onesweep_tile<K, false, BLOCK, KPT>(sm, sk, dk, nullptr, nullptr, base, cnt, shift, tile, first, seg_base,
-                                        seg_hist + ((size_t)c * DIG + pass) * 256, seg_base, lb_cur, lb_next,
-                                        spin_limit);
+    onesweep_tile<K, false, BLOCK, KPT, RB>(sm, sk, dk, nullptr, nullptr, base, cnt, shift, tile, first, seg_base,
+                                            seg_hist + ((size_t)c * DIG + pass) * 256, seg_base, lb_cur, lb_next,
+                                            spin_limit);
 }
 
 // Fix-ups after a segmented scan: empty segments get the base of the next
@@ -418,10 +418,16 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
 template <typename K, int BLOCK, int KPT>
 void launch_seg_pass(tc_batch* b, hipStream_t s, uint32_t grid, const K* src, K* dst, uint32_t pass) {
     const Plan p = plan_view(b);
-    uint32_t* lb0 = b->lb + (size_t)(pass & 1) * b->cap_tiles * 256;
-    uint32_t* lb1 = b->lb + (size_t)((pass + 1) & 1) * b->cap_tiles * 256;
-    hipLaunchKernelGGL((k_seg_onesweep<K, BLOCK, KPT>), dim3(grid), dim3(BLOCK), 0, s, p, src, dst, 8u * pass, pass,
-                       b->seg_hist, lb0, (pass + 1 < sizeof(K)) ? lb1 : nullptr, b->ctx->spin_limit);
+    uint32_t* lb0 = b->lb + (size_t)(pass & 1) * b->cap_tiles * LB_ROW;
+    uint32_t* lb1 = b->lb + (size_t)((pass + 1) & 1) * b->cap_tiles * LB_ROW;
+    uint32_t* nxt = (pass + 1 < sizeof(K)) ? lb1 : nullptr;
+    // The codes are Morton codes: the top digit has fewer live bits.
+    if (pass + 1 == sizeof(K))
+        hipLaunchKernelGGL((k_seg_onesweep<K, BLOCK, KPT, morton_top_digit_bits<K>()>), dim3(grid), dim3(BLOCK), 0, s,
+                           p, src, dst, 8u * pass, pass, b->seg_hist, lb0, nxt, b->ctx->spin_limit);
+    else
+        hipLaunchKernelGGL((k_seg_onesweep<K, BLOCK, KPT, 8>), dim3(grid), dim3(BLOCK), 0, s, p, src, dst, 8u * pass,
+                           pass, b->seg_hist, lb0, nxt, b->ctx->spin_limit);
 }
 
 template <typename K>

@@ -186,12 +186,12 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_morton_hist(
     hist_merge<DIG>(&s_h[0][0], acc);
 }
 
-template <typename K, bool PAIRS, int BLOCK, int KPT>
+template <typename K, bool PAIRS, int BLOCK, int KPT, int RB>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(
     const K* __restrict__ sk, K* __restrict__ dk, const uint32_t* __restrict__ sv,
     uint32_t* __restrict__ dv, uint32_t n, uint32_t shift, const uint32_t* __restrict__ digit_prefix,
     uint32_t* __restrict__ lb_cur, uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
-    using Sh = OnesweepShared<K, BLOCK, KPT>;
+    using Sh = OnesweepShared<K, BLOCK, KPT, RB>;
     __shared__ Sh sm;
     const uint32_t num_tiles = ceil_div_u32(n, Sh::TILE);
     // One tile per workgroup, tile = workgroup id: no claim counter (claims on
@@ -202,27 +202,32 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     if (tile >= num_tiles) return;
     const uint32_t base = tile * Sh::TILE;
     const uint32_t cnt = n - base < Sh::TILE ? n - base : Sh::TILE;
-    onesweep_tile<K, PAIRS, BLOCK, KPT>(sm, sk, dk, sv, dv, base, cnt, shift, tile, 0u, 0u, digit_prefix, 0u, lb_cur,
-                                        lb_next, spin_limit);
+    onesweep_tile<K, PAIRS, BLOCK, KPT, RB>(sm, sk, dk, sv, dv, base, cnt, shift, tile, 0u, 0u, digit_prefix, 0u,
+                                            lb_cur, lb_next, spin_limit);
 }
 
+// narrow: the pass ranks only the top digit's live Morton-code bits.
 template <typename K, bool PAIRS, int BLOCK, int KPT>
 void launch_pass(uint32_t tiles, hipStream_t s, const K* sk, K* dk, const uint32_t* sv, uint32_t* dv,
                  uint32_t n, uint32_t shift, const uint32_t* prefix, uint32_t* lb_cur, uint32_t* lb_next,
-                 uint32_t spin_limit) {
-    hipLaunchKernelGGL((k_onesweep<K, PAIRS, BLOCK, KPT>), dim3(tiles), dim3(BLOCK), 0, s, sk, dk, sv, dv,
-                       n, shift, prefix, lb_cur, lb_next, spin_limit);
+                 uint32_t spin_limit, bool narrow) {
+    if (narrow)
+        hipLaunchKernelGGL((k_onesweep<K, PAIRS, BLOCK, KPT, morton_top_digit_bits<K>()>), dim3(tiles), dim3(BLOCK),
+                           0, s, sk, dk, sv, dv, n, shift, prefix, lb_cur, lb_next, spin_limit);
+    else
+        hipLaunchKernelGGL((k_onesweep<K, PAIRS, BLOCK, KPT, 8>), dim3(tiles), dim3(BLOCK), 0, s, sk, dk, sv, dv,
+                           n, shift, prefix, lb_cur, lb_next, spin_limit);
 }
 
 template <typename K, bool PAIRS>
 int dispatch_pass(tc_ctx* ctx, SortShape sh, uint32_t tiles, hipStream_t s, const K* sk, K* dk,
                   const uint32_t* sv, uint32_t* dv, uint32_t n, uint32_t shift, const uint32_t* prefix,
-                  uint32_t* lb_cur, uint32_t* lb_next) {
+                  uint32_t* lb_cur, uint32_t* lb_next, bool narrow = false) {
     ProfScope prof(ctx, s, K_BINNING);
 #define TC_PASS_CASE(B, KP)                                                                         \
     if (sh.block == B && sh.kpt == KP) {                                                            \
         launch_pass<K, PAIRS, B, KP>(tiles, s, sk, dk, sv, dv, n, shift, prefix, lb_cur, lb_next,   \
-                                     ctx->spin_limit);                                              \
+                                     ctx->spin_limit, narrow);                                      \
         TC_LAUNCH_CHECK();                                                                           \
         return TC_OK;                                                                                \
     }
@@ -319,12 +324,14 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
     for (int p = 0; p < DIG; ++p) {
         uint32_t* cur = ss.region[p & 1];
         uint32_t* nxt = (p + 1 < DIG) ? ss.region[(p + 1) & 1] : nullptr;
+        // Morton codes (computed here from pts) have a narrow top digit.
+        const bool narrow = pts != nullptr && p == DIG - 1;
         if (vals) {
             TC_TRY((dispatch_pass<K, true>(ctx, sh, tiles, s, src, dst, vsrc, vdst, n, 8u * p, ss.hist + 256 * p,
-                                           cur, nxt)));
+                                           cur, nxt, narrow)));
         } else {
             TC_TRY((dispatch_pass<K, false>(ctx, sh, tiles, s, src, dst, nullptr, nullptr, n, 8u * p,
-                                            ss.hist + 256 * p, cur, nxt)));
+                                            ss.hist + 256 * p, cur, nxt, narrow)));
         }
         K* t = src; src = dst; dst = t;
         uint32_t* tv = vsrc; vsrc = vdst; vdst = tv;

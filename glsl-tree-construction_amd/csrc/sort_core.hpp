@@ -8,18 +8,35 @@
 
 #include "tc_internal.hpp"
 
-// Peer mask: lanes of this wave whose 8-bit digit equals mine, among `live`
-// lanes (8 ballots; the reference's 32-lane version is
+// Digit width: 8 bits per pass, except that the last pass over Morton codes,
+// whose top digit has fewer live bits (30-bit codes: bits 24..29; 63-bit
+// codes: bits 56..62), ranks only those bits.  Fewer bits mean fewer ballots
+// per key and fewer digits to publish and look back over (measured on MI355X,
+// 2^26 keys: 0.189 ms for an 8-bit pass, 0.175 ms for 7 bits, 0.164 ms for 6).
+// The digit histogram and the look-back rows keep 256 words per digit
+// position / tile whatever the width; a narrow pass uses the first 2^RB.
+constexpr uint32_t LB_ROW = 256;
+
+// Live bits of the top digit (shift 8*(bytes-1)) of a Morton code: 30 - 24
+// for 30-bit codes, 63 - 56 for 63-bit codes.
+template <typename K>
+constexpr int morton_top_digit_bits() {
+    return sizeof(K) == 4 ? 6 : 7;
+}
+
+// Peer mask: lanes of this wave whose RB-bit digit equals mine, among `live`
+// lanes (RB ballots; the reference's 32-lane 8-bit version is
 // shader/new_radix_sort.comp:147-152).  Must be called with all lanes active.
 //
 // Per bit b: rep = 0 or ~0 (my bit b, sign-extended), m = ballot of bit b;
 // rep ^ m has a 1 for every lane whose bit b differs from mine.  OR-ing those
-// over the 8 bits gives the lanes with a different digit (xor + or3 per half
-// word, about 40 VALU ops per key instead of select-and-and chains).
-TC_DEV uint64_t match_digit8(uint32_t d, uint64_t live) {
+// over the RB bits gives the lanes with a different digit (xor + or3 per half
+// word, about 5 VALU ops per bit instead of select-and-and chains).
+template <int RB>
+TC_DEV uint64_t match_digit(uint32_t d, uint64_t live) {
     uint32_t diff_lo = 0u, diff_hi = 0u;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < RB; ++b) {
         uint32_t rep = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);
         // Opaque to the optimiser, which would otherwise test the bit again
         // with a shift + compare instead of comparing rep (no instruction).
@@ -32,22 +49,23 @@ TC_DEV uint64_t match_digit8(uint32_t d, uint64_t live) {
     return live & ~(((uint64_t)diff_hi << 32) | diff_lo);
 }
 
-template <typename K>
+template <typename K, int RB = 8>
 TC_DEV uint32_t digit_of(K k, uint32_t shift) {
-    return (uint32_t)((k >> shift) & 255u);
+    return (uint32_t)((k >> shift) & ((1u << RB) - 1u));
 }
 
-template <typename K, int BLOCK, int KPT>
+template <typename K, int BLOCK, int KPT, int RB = 8>
 struct OnesweepShared {
     static constexpr int WAVES = BLOCK / 64;
     static constexpr uint32_t TILE = BLOCK * KPT;
+    static constexpr int D = 1 << RB;
     // Per-wave digit counters, then tile-local positions (< TILE <= 65535,
     // so 16 bits); the fall-back digit counts reuse the space afterwards.
     union {
-        uint16_t whist[WAVES * 256];
-        uint32_t fb[256];
+        uint16_t whist[WAVES * D];
+        uint32_t fb[D];
     };
-    uint32_t gbase[256];
+    uint32_t gbase[D];
     uint32_t tmp[WAVES + 2];
     uint32_t tile;
     uint32_t aux;
@@ -55,24 +73,28 @@ struct OnesweepShared {
     alignas(16) unsigned char stage[TILE * sizeof(K)];
 };
 
-// One digit-binning tile step (shader/new_radix_sort.comp:116-314):
+// One digit-binning tile step (shader/new_radix_sort.comp:116-314), digit
+// d = (key >> shift) & (2^RB - 1):
 //   keys sk[base, base+cnt) (cnt <= TILE) go to
 //   dk[prefix[d] + add + (#digit-d keys of earlier tiles of this segment) + rank].
-// `tile` indexes the look-back rows (256 u32 words per tile) of lb_cur; the
+// The caller guarantees that the key bits above shift + RB are equal within
+// the segment (zero for Morton codes), so RB < 8 sorts like an 8-bit digit.
+// `tile` indexes the look-back rows (LB_ROW u32 words per tile) of lb_cur; the
 // look-back walks down to `first`, the first tile of the segment, which
 // publishes its counts as inclusive.  lb_next (may be NULL) is the next pass's
 // look-back row of this tile, cleared here.
 // seg_base is the index of the segment's first key (the fall-back counts the
 // keys in [seg_base, base)); spin_limit bounds the look-back wait.
-template <typename K, bool PAIRS, int BLOCK, int KPT, bool FULL>
-TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict__ sk, K* __restrict__ dk,
+template <typename K, bool PAIRS, int BLOCK, int KPT, int RB, bool FULL>
+TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* __restrict__ sk, K* __restrict__ dk,
                                const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
                                uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first, uint32_t seg_base,
                                const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
                                uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
     constexpr int WAVES = BLOCK / 64;
     constexpr uint32_t WAVE_TILE = 64 * KPT;
-    static_assert(BLOCK >= 256, "one thread per digit");
+    constexpr uint32_t D = 1u << RB;
+    static_assert(RB >= 1 && RB <= 8 && BLOCK >= (int)D, "one thread per digit");
     K* s_keys = reinterpret_cast<K*>(sm.stage);
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(sm.stage);  // reused once the keys are out
 
@@ -83,7 +105,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
     const uint32_t wofs = wave * WAVE_TILE + lane;  // tile-relative index of key 0 of this lane
 
     static_assert(BLOCK * KPT <= 65535, "tile positions must fit the 16-bit counters");
-    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) sm.whist[i] = 0u;
+    for (uint32_t i = tid; i < WAVES * D; i += BLOCK) sm.whist[i] = 0u;
 
     K keys[KPT];
     uint32_t vals[KPT];
@@ -104,7 +126,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
         }
     }
     if (lb_next) {
-        for (uint32_t i = tid; i < 256; i += BLOCK) lb_next[tile * 256 + i] = 0u;
+        for (uint32_t i = tid; i < LB_ROW; i += BLOCK) lb_next[tile * LB_ROW + i] = 0u;
     }
     __syncthreads();  // counters cleared
 
@@ -113,12 +135,12 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
     // observes the leader updates of iteration i.  The signal fence keeps the
     // compiler from moving the LDS accesses across iterations (it emits no
     // instruction; a volatile pointer here would turn them into flat accesses).
-    uint16_t* wh = sm.whist + wave * 256;
+    uint16_t* wh = sm.whist + wave * D;
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
-        const uint32_t d = digit_of<K>(keys[i], shift);
+        const uint32_t d = digit_of<K, RB>(keys[i], shift);
         const bool valid = full || (wofs + i * 64 < cnt);
-        const uint64_t peers = match_digit8(d, __ballot(valid));
+        const uint64_t peers = match_digit<RB>(d, __ballot(valid));
         const uint32_t lt = mbcnt64(peers);
         const uint32_t before = wh[d];
         pos[i] = before + lt;
@@ -129,23 +151,23 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
 
     // Per digit: exclusive prefix over waves, tile total, publish aggregate.
     uint32_t total_d = 0;
-    if (tid < 256) {
+    if (tid < D) {
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) {
-            const uint32_t c = sm.whist[w * 256 + tid];
-            sm.whist[w * 256 + tid] = (uint16_t)run;
+            const uint32_t c = sm.whist[w * D + tid];
+            sm.whist[w * D + tid] = (uint16_t)run;
             run += c;
         }
         total_d = run;
-        st_agent(&lb_cur[tile * 256 + tid], (tile == first ? LB_INCLUSIVE : LB_AGGREGATE) | (total_d << 2));
+        st_agent(&lb_cur[tile * LB_ROW + tid], (tile == first ? LB_INCLUSIVE : LB_AGGREGATE) | (total_d << 2));
     }
     uint32_t tile_total;
     const uint32_t start = block_exclusive_scan<BLOCK>(total_d, sm.tmp, &tile_total);
-    if (tid < 256) {
+    if (tid < D) {
         // whist[w][d] becomes the tile-local position of wave w's first digit-d key.
 #pragma unroll
-        for (int w = 0; w < WAVES; ++w) sm.whist[w * 256 + tid] = (uint16_t)(sm.whist[w * 256 + tid] + start);
+        for (int w = 0; w < WAVES; ++w) sm.whist[w * D + tid] = (uint16_t)(sm.whist[w * D + tid] + start);
     }
     __syncthreads();
 
@@ -154,8 +176,8 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
     for (int i = 0; i < KPT; ++i) {
         const bool valid = full || (wofs + i * 64 < cnt);
         if (valid) {
-            const uint32_t d = digit_of<K>(keys[i], shift);
-            const uint32_t lp = sm.whist[wave * 256 + d] + pos[i];
+            const uint32_t d = digit_of<K, RB>(keys[i], shift);
+            const uint32_t lp = sm.whist[wave * D + d] + pos[i];
             s_keys[lp] = keys[i];
             pos[i] = lp;
         }
@@ -171,13 +193,13 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
     if (tid == 0) sm.fallback = (spin_limit == 0u && tile != first) ? 1u : 0u;  // 0: always count directly
     uint32_t excl = 0;
     __syncthreads();
-    if (tid < 256 && tile != first && spin_limit != 0u) {
+    if (tid < D && tile != first && spin_limit != 0u) {
         uint32_t k = tile - 1;
         uint32_t spins = 0;
         while (true) {
             uint32_t v[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = (k >= first + j) ? ld_acquire_agent(&lb_cur[(k - j) * 256 + tid]) : 0u;
+            for (int j = 0; j < 4; ++j) v[j] = (k >= first + j) ? ld_acquire_agent(&lb_cur[(k - j) * LB_ROW + tid]) : 0u;
             bool found = false;
             uint32_t used = 0;
 #pragma unroll
@@ -202,14 +224,14 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
     }
     __syncthreads();
     if (sm.fallback) {
-        for (uint32_t i = tid; i < 256; i += BLOCK) sm.fb[i] = 0u;
+        for (uint32_t i = tid; i < D; i += BLOCK) sm.fb[i] = 0u;
         __syncthreads();
-        for (uint32_t i = seg_base + tid; i < base; i += BLOCK) atomicAdd(&sm.fb[digit_of<K>(sk[i], shift)], 1u);
+        for (uint32_t i = seg_base + tid; i < base; i += BLOCK) atomicAdd(&sm.fb[digit_of<K, RB>(sk[i], shift)], 1u);
         __syncthreads();
-        if (tid < 256) excl = sm.fb[tid];
+        if (tid < D) excl = sm.fb[tid];
     }
-    if (tid < 256) {
-        if (tile != first) st_agent(&lb_cur[tile * 256 + tid], LB_INCLUSIVE | ((excl + total_d) << 2));
+    if (tid < D) {
+        if (tile != first) st_agent(&lb_cur[tile * LB_ROW + tid], LB_INCLUSIVE | ((excl + total_d) << 2));
         sm.gbase[tid] = prefix[tid] + add + excl - start;
     }
     __syncthreads();
@@ -220,7 +242,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
         const uint32_t j = tid + i * BLOCK;
         if (full || j < cnt) {
             const K key = s_keys[j];
-            dst[i] = sm.gbase[digit_of<K>(key, shift)] + j;
+            dst[i] = sm.gbase[digit_of<K, RB>(key, shift)] + j;
             dk[dst[i]] = key;
         }
     }
@@ -241,16 +263,16 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT>& sm, const K* __res
 }
 
 // Full tiles take the variant without bounds tests (cnt is uniform per workgroup).
-template <typename K, bool PAIRS, int BLOCK, int KPT>
-TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT>& sm, const K* __restrict__ sk, K* __restrict__ dk,
+template <typename K, bool PAIRS, int BLOCK, int KPT, int RB>
+TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* __restrict__ sk, K* __restrict__ dk,
                           const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
                           uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first, uint32_t seg_base,
                           const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
                           uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
-    if (cnt == OnesweepShared<K, BLOCK, KPT>::TILE)
-        onesweep_tile_impl<K, PAIRS, BLOCK, KPT, true>(sm, sk, dk, sv, dv, base, cnt, shift, tile, first, seg_base,
-                                                       prefix, add, lb_cur, lb_next, spin_limit);
+    if (cnt == OnesweepShared<K, BLOCK, KPT, RB>::TILE)
+        onesweep_tile_impl<K, PAIRS, BLOCK, KPT, RB, true>(sm, sk, dk, sv, dv, base, cnt, shift, tile, first,
+                                                           seg_base, prefix, add, lb_cur, lb_next, spin_limit);
     else
-        onesweep_tile_impl<K, PAIRS, BLOCK, KPT, false>(sm, sk, dk, sv, dv, base, cnt, shift, tile, first,
-                                                        seg_base, prefix, add, lb_cur, lb_next, spin_limit);
+        onesweep_tile_impl<K, PAIRS, BLOCK, KPT, RB, false>(sm, sk, dk, sv, dv, base, cnt, shift, tile, first,
+                                                            seg_base, prefix, add, lb_cur, lb_next, spin_limit);
 }
