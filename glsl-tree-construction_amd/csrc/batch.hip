@@ -441,14 +441,15 @@ int seg_pass(tc_batch* b, hipStream_t s, const K* src, K* dst, uint32_t pass) {
         return TC_OK;                                                 \
     }
     TC_SEG_CASE(256, 8)
-    TC_SEG_CASE(256, 12)
-    TC_SEG_CASE(256, 16)
-    TC_SEG_CASE(256, 20)
     TC_SEG_CASE(512, 8)
     TC_SEG_CASE(512, 12)
     TC_SEG_CASE(512, 16)
     TC_SEG_CASE(512, 20)
     TC_SEG_CASE(512, 24)
+    TC_SEG_CASE(512, 28)
+    TC_SEG_CASE(512, 32)
+    TC_SEG_CASE(512, 36)
+    TC_SEG_CASE(1024, 16)
 #undef TC_SEG_CASE
     return TC_ERR_INVALID;
 }

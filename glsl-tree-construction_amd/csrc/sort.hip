@@ -232,14 +232,15 @@ int dispatch_pass(tc_ctx* ctx, SortShape sh, uint32_t tiles, hipStream_t s, cons
         return TC_OK;                                                                                \
     }
     TC_PASS_CASE(256, 8)
-    TC_PASS_CASE(256, 12)
-    TC_PASS_CASE(256, 16)
-    TC_PASS_CASE(256, 20)
     TC_PASS_CASE(512, 8)
     TC_PASS_CASE(512, 12)
     TC_PASS_CASE(512, 16)
     TC_PASS_CASE(512, 20)
     TC_PASS_CASE(512, 24)
+    TC_PASS_CASE(512, 28)
+    TC_PASS_CASE(512, 32)
+    TC_PASS_CASE(512, 36)
+    TC_PASS_CASE(1024, 16)
 #undef TC_PASS_CASE
     return TC_ERR_INVALID;
 }

@@ -122,9 +122,9 @@ size_t tc_sort_scratch_bytes(uint32_t n, int passes, int tile);
 struct SortShape {
     int block, kpt;
 };
-constexpr int N_SORT_SHAPES = 9;
-constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {{256, 8}, {256, 12}, {256, 16}, {256, 20}, {512, 8},
-                                                  {512, 12}, {512, 16}, {512, 20}, {512, 24}};
+constexpr int N_SORT_SHAPES = 10;
+constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {{256, 8},  {512, 8},  {512, 12}, {512, 16}, {512, 20},
+                                                  {512, 24}, {512, 28}, {512, 32}, {512, 36}, {1024, 16}};
 // The shape a sort of n keys uses: the tuned one for the nearest measured size
 // when the context has been tuned, else the context's fixed shape.
 SortShape tc_sort_shape(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n);

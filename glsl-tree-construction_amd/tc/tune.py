@@ -1,6 +1,6 @@
 """Autotuner of the sort's launch shape (tc_autotune_sort).
 
-The binning pass is built for nine (workgroup size, keys per lane) shapes;
+The binning pass is built for ten (workgroup size, keys per lane) shapes;
 which is fastest depends on the key width, the payload and the size.
 ``autotune`` times them all on the device and the context then picks, for each
 sort, the shape measured at the nearest size (batches: at their average cloud

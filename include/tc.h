@@ -76,8 +76,8 @@ int tc_destroy(tc_ctx* ctx);
  * calls allocate nothing (needed before stream capture). */
 int tc_reserve(tc_ctx* ctx, uint32_t max_n);
 /* Launch parameters of the sort's binning pass, one of the compiled shapes
- * (block, keys per thread): 256 x {8, 12, 16, 20} and 512 x {8, 12, 16, 20,
- * 24}.  0 keeps the current value. */
+ * (block, keys per thread): 256 x 8, 512 x {8, 12, 16, 20, 24, 28, 32, 36} and
+ * 1024 x 16.  0 keeps the current value. */
 int tc_set_sort_params(tc_ctx* ctx, int block, int keys_per_thread);
 int tc_get_sort_params(tc_ctx* ctx, int* block, int* keys_per_thread);
 /* Autotuner (no reference counterpart): times every binning-pass launch shape

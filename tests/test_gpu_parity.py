@@ -155,8 +155,8 @@ def test_decode_kernel(gpu_ctx, oracle):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("block,kpt", [(256, 8), (256, 12), (256, 16), (256, 20), (512, 8), (512, 12),
-                                       (512, 16), (512, 20), (512, 24)])
+@pytest.mark.parametrize("block,kpt", [(256, 8), (512, 8), (512, 12), (512, 16), (512, 20), (512, 24), (512, 28),
+                                       (512, 32), (512, 36), (1024, 16)])
 def test_sort_launch_shapes(gpu_ctx, oracle, block, kpt):
     """Every binning launch shape the tuner may pick gives the stable order."""
     r = np.random.default_rng(block * 100 + kpt)

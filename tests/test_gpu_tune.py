@@ -39,7 +39,7 @@ def test_autotune_table_and_sorts(oracle, tclib, tmp_path):
         tc.tune.load(ctx, path)
         assert tc.tune.table(ctx, 1 << 20) == t
         # An explicit shape replaces the tuned ones.
-        ctx.set_sort_params(256, 12)
-        assert tc.tune.shape(ctx, 64, True, 1 << 16) == (256, 12)
+        ctx.set_sort_params(512, 28)
+        assert tc.tune.shape(ctx, 64, True, 1 << 16) == (512, 28)
     finally:
         ctx.close()
