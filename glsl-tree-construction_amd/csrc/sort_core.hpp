@@ -30,14 +30,16 @@ constexpr int morton_top_digit_bits() {
 //
 // Per bit b: rep = 0 or ~0 (my bit b, sign-extended), m = ballot of bit b;
 // rep ^ m has a 1 for every lane whose bit b differs from mine.  OR-ing those
-// over the RB bits gives the lanes with a different digit (xor + or3 per half
-// word, about 5 VALU ops per bit instead of select-and-and chains).
-template <int RB>
-TC_DEV uint64_t match_digit(uint32_t d, uint64_t live) {
+// over the RB bits gives the lanes with a different digit: 4 VALU ops per bit
+// (bit extract, compare, one v_bitop3 per half word) instead of
+// select-and-and chains.  The digit is bits [off, off + RB) of `word` (off is uniform, so off + b is a
+// scalar operand of each bit extract and the key needs no shift per key).
+template <int RB, bool ALL_LIVE = false>
+TC_DEV uint64_t match_digit(uint32_t word, uint32_t off, uint64_t live) {
     uint32_t diff_lo = 0u, diff_hi = 0u;
 #pragma unroll
-    for (int b = 0; b < RB; ++b) {
-        uint32_t rep = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);
+    for (int b = 0; b < RB - 1; ++b) {
+        uint32_t rep = (uint32_t)__builtin_amdgcn_sbfe((int32_t)word, off + b, 1);
         // Opaque to the optimiser, which would otherwise test the bit again
         // with a shift + compare instead of comparing rep (no instruction).
         asm("" : "+v"(rep));
@@ -46,12 +48,32 @@ TC_DEV uint64_t match_digit(uint32_t d, uint64_t live) {
         diff_lo = __builtin_amdgcn_bitop3_b32(rep, diff_lo, (uint32_t)m, 0xde);
         diff_hi = __builtin_amdgcn_bitop3_b32(rep, diff_hi, (uint32_t)(m >> 32), 0xde);
     }
-    return live & ~(((uint64_t)diff_hi << 32) | diff_lo);
+    uint32_t rep = (uint32_t)__builtin_amdgcn_sbfe((int32_t)word, off + RB - 1, 1);
+    asm("" : "+v"(rep));
+    const uint64_t m = __ballot(rep != 0u);
+    if constexpr (ALL_LIVE) {
+        // Every lane live: the last bit's step also complements, ~((rep ^ m) | diff) (table 0x21).
+        const uint32_t lo = __builtin_amdgcn_bitop3_b32(rep, diff_lo, (uint32_t)m, 0x21);
+        const uint32_t hi = __builtin_amdgcn_bitop3_b32(rep, diff_hi, (uint32_t)(m >> 32), 0x21);
+        return ((uint64_t)hi << 32) | lo;
+    } else {
+        diff_lo = __builtin_amdgcn_bitop3_b32(rep, diff_lo, (uint32_t)m, 0xde);
+        diff_hi = __builtin_amdgcn_bitop3_b32(rep, diff_hi, (uint32_t)(m >> 32), 0xde);
+        return live & ~(((uint64_t)diff_hi << 32) | diff_lo);
+    }
 }
 
 template <typename K, int RB = 8>
 TC_DEV uint32_t digit_of(K k, uint32_t shift) {
     return (uint32_t)((k >> shift) & ((1u << RB) - 1u));
+}
+
+// The 32-bit half of a key that holds the digit at `shift` (a multiple of 8,
+// so with RB <= 8 the digit never straddles the halves).
+template <typename K>
+TC_DEV uint32_t digit_word(K k, uint32_t shift) {
+    if constexpr (sizeof(K) == 4) return (uint32_t)k;
+    else return shift >= 32u ? (uint32_t)(k >> 32) : (uint32_t)k;
 }
 
 template <typename K, int BLOCK, int KPT, int RB = 8>
@@ -138,9 +160,10 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
     uint16_t* wh = sm.whist + wave * D;
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
-        const uint32_t d = digit_of<K, RB>(keys[i], shift);
+        const uint32_t word = digit_word<K>(keys[i], shift);
+        const uint32_t d = __builtin_amdgcn_ubfe(word, shift & 31u, RB);
         const bool valid = full || (wofs + i * 64 < cnt);
-        const uint64_t peers = match_digit<RB>(d, __ballot(valid));
+        const uint64_t peers = match_digit<RB, FULL>(word, shift & 31u, FULL ? ~0ull : __ballot(valid));
         const uint32_t lt = mbcnt64(peers);
         const uint32_t before = wh[d];
         pos[i] = before + lt;
