@@ -362,3 +362,32 @@ def test_markers_do_not_change_results(gpu_ctx, oracle, tclib):
         if k in want and k in got:
             assert np.array_equal(got[k], want[k]), k
     assert np.array_equal(got["nodes"].view(np.uint8), want["nodes"].view(np.uint8))
+
+
+@pytest.mark.parametrize("n", [1, 2, 8191, 8192, 8193, 16383, 16384, 16385, 3 * 16384 + 5, 300_001])
+def test_scan_modes_at_tile_edges(gpu_ctx, oracle, n):
+    """Unique (u32, u64), exclusive scan and the fused edge-count scan at sizes
+    around multiples of the scan kernels' 8192-element tile, against the
+    oracle."""
+    r = np.random.default_rng(n + 17)
+    for dt in (np.uint32, np.uint64):
+        keys = np.sort(r.integers(0, max(n // 3, 2), size=n, dtype=np.uint64).astype(dt))
+        out, cnt = empty(n, dt), empty(1, np.uint32)
+        gpu_ctx.unique(dev(keys), out, cnt)
+        want = oracle.unique(keys)
+        got_n = int(host(cnt, np.uint32)[0])
+        assert got_n == len(want) and np.array_equal(host(out, dt)[:got_n], want)
+    vals = r.integers(0, 1000, size=n, dtype=np.uint64).astype(np.uint32)
+    out = empty(n + 1, np.uint32)
+    gpu_ctx.exclusive_scan(dev(vals), out)
+    assert np.array_equal(host(out, np.uint32), oracle.exclusive_scan(vals))
+    if n >= 2:
+        codes = np.unique(r.integers(0, 1 << 30, size=n + 64, dtype=np.uint64).astype(np.uint32))[:n]
+        t = oracle.build_radix_tree(codes)
+        nb = len(codes) - 1
+        if nb >= 1:
+            ec, offs = empty(nb, np.uint32), empty(nb + 1, np.uint32)
+            gpu_ctx.edge_count_scan(dev(t["prefix_n"]), dev(t["parent"]), ec, offs, nb)
+            want_ec = oracle.edge_count(t["prefix_n"], t["parent"])
+            assert np.array_equal(host(ec, np.uint32), want_ec)
+            assert np.array_equal(host(offs, np.uint32), oracle.exclusive_scan(want_ec))

@@ -51,7 +51,7 @@ def test_indexed_build_matches_oracle(gpu_ctx, oracle, tclib, bits, kind, n):
 
 
 @pytest.mark.parametrize("dt", [np.uint32, np.uint64])
-@pytest.mark.parametrize("n", [1, 8191, 8192, 8193, 3 * 8192 + 5, 200_000])
+@pytest.mark.parametrize("n", [1, 8191, 8192, 8193, 3 * 8192 + 5, 16383, 16384, 16385, 3 * 16384 + 5, 200_000])
 def test_unique_runs_matches_oracle(gpu_ctx, oracle, dt, n):
     r = np.random.default_rng(n)
     keys = np.sort(r.integers(0, max(n // 7, 2), size=n, dtype=np.uint64).astype(dt))
