@@ -536,7 +536,11 @@ extern "C" int tc_batch_create(tc_ctx* ctx, uint32_t cap_clouds, uint32_t cap_po
     b->cap_points = cap_points;
     b->code_bits = code_bits;
     // Launch shape of the binning passes: the tuned one for the average cloud.
-    const SortShape sh = tc_sort_shape(ctx, code_bits == 30 ? 4 : 8, false, cap_points / cap_clouds);
+    // One segmented launch sorts every cloud's keys: the shape tuned for the
+    // total key count (measured: 1,024 clouds of 140.9 M points in all take
+    // 7.41 ms per replay with 512 x 20 or larger tiles, 7.86 ms with the
+    // 512 x 8 tiles tuned for their average size).
+    const SortShape sh = tc_sort_shape(ctx, code_bits == 30 ? 4 : 8, false, cap_points);
     b->block = sh.block;
     b->kpt = sh.kpt;
     b->tile = (uint32_t)(b->block * b->kpt);

@@ -3,8 +3,8 @@
 The binning pass is built for ten (workgroup size, keys per lane) shapes;
 which is fastest depends on the key width, the payload and the size.
 ``autotune`` times them all on the device and the context then picks, for each
-sort, the shape measured at the nearest size (batches: at their average cloud
-size).  The table can be saved as JSON and restored into another context.
+sort, the shape measured at the nearest size (batches: at their total point
+capacity).  The table can be saved as JSON and restored into another context.
 """
 from __future__ import annotations
 

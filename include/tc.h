@@ -83,7 +83,7 @@ int tc_get_sort_params(tc_ctx* ctx, int* block, int* keys_per_thread);
 /* Autotuner (no reference counterpart): times every binning-pass launch shape
  * on random keys at sizes 2^12, 2^14, ... up to max_n, for u32 and u64 keys with
  * and without a u32 payload, and from then on each sort uses the fastest shape
- * measured at the size nearest its own (batches: at their average cloud size).
+ * measured at the size nearest its own (batches: at their total point capacity).
  * Blocking.  tc_set_sort_params switches back to one fixed shape. */
 int tc_autotune_sort(tc_ctx* ctx, uint32_t max_n);
 /* The shape a sort of n keys (key_bits 32 or 64, pairs 0/1) would use now. */
