@@ -120,6 +120,25 @@ TC_DEV uint32_t seg_search(const uint32_t* off, uint32_t lo, uint32_t n, uint32_
 
 TC_DEV uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t x) { return seg_search(off, 0, n, x); }
 
+// seg_find by a whole wave (all 64 lanes active, same x): each round tests 64
+// evenly spaced candidates at once, so 1,024 segments take 2 dependent loads
+// instead of 10.  off[c] <= x holds for a prefix of c (off is nondecreasing),
+// so the ballot is a run of low lanes; the last of them brackets the answer.
+TC_DEV uint32_t seg_find_wave(const uint32_t* off, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;  // answer in [lo, hi), off[lo] <= x
+    const uint32_t lane = __lane_id();
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t k = lo + lane * step;
+        const bool le = k < hi && off[k] <= x;
+        const uint64_t m = __ballot(le);
+        const uint32_t last = 63u - (uint32_t)__builtin_clzll(m);  // lane 0 always holds: off[lo] <= x
+        lo = lo + last * step;
+        hi = lo + step < hi ? lo + step : hi;
+    }
+    return lo;
+}
+
 // As seg_find, starting from c0, the segment of an index <= x (a tile's first
 // slot): a few steps forward cover the usual case of a tile inside one or two
 // clouds, a binary search the rest.
@@ -286,10 +305,16 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* _
     // long ranges afterwards (see k_build_radix_tree).  With per-cloud output
     // buffers (tc_batch_set_io) the unique codes and radix arrays are also
     // stored there.
-    __shared__ K s_win[SEG_BRT_TILE + 2 * SEG_SEARCH_HALO];
+    // s_win: the concatenated codes of [ws, ws + W) (all-ones outside the
+    // array); s_fast: the same with all-ones outside the tile's first cloud, so
+    // that cloud's nodes search without bounds checks (as k_build_radix_tree).
+    constexpr int W = SEG_BRT_TILE + 2 * SEG_SEARCH_HALO;
+    static_assert(SEG_SEARCH_HALO >= (int)SEG_BRT_LIMIT, "bounded searches must stay inside the window");
+    __shared__ K s_win[W];
+    __shared__ K s_fast[W];
     __shared__ uint32_t s_long[SEG_BRT_TILE];
     __shared__ uint32_t s_nlong;
-    __shared__ uint32_t s_c0;
+    __shared__ uint32_t s_c0, s_b0, s_e0;
     const uint32_t C = p.desc[0];
     const bool to_clouds = (p.desc[3] & PLAN_OUT_PTRS) != 0;
     const uint32_t total = ubase[C];
@@ -301,28 +326,54 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* _
     };
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const int64_t t0 = (int64_t)t * SEG_BRT_TILE;
-        const int64_t w0 = t0 > SEG_SEARCH_HALO ? t0 - SEG_SEARCH_HALO : 0;
-        const int64_t w1 = t0 + SEG_BRT_TILE + SEG_SEARCH_HALO < total ? t0 + SEG_BRT_TILE + SEG_SEARCH_HALO : total;
+        const int64_t ws = t0 - SEG_SEARCH_HALO;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            s_c0 = seg_find(ubase, C, (uint32_t)t0);
-            s_nlong = 0;
+        if (threadIdx.x < 64) {
+            // The cloud of the tile's first slot and its slot range [s_b0, s_e0).
+            const uint32_t c = seg_find_wave(ubase, C, (uint32_t)t0);
+            if (threadIdx.x == 0) {
+                s_c0 = c;
+                s_b0 = ubase[c];
+                s_e0 = ubase[c + 1];
+                s_nlong = 0;
+            }
         }
-        stage_codes<SEG_BLOCK, SEG_BRT_TILE + 2 * SEG_SEARCH_HALO>(s_win, uniq, w0, w0, w1, (K)~(K)0);
+        stage_codes<SEG_BLOCK, W>(s_win, uniq, ws, 0, total, (K)~(K)0);
         __syncthreads();
-        const uint32_t c0 = s_c0;
+        const uint32_t c0 = s_c0, b0 = s_b0, e0 = s_e0;
+        for (uint32_t q = threadIdx.x; q < (uint32_t)W; q += SEG_BLOCK) {
+            const int64_t g = ws + q;
+            s_fast[q] = (g >= (int64_t)b0 && g < (int64_t)e0) ? s_win[q] : (K)~(K)0;
+        }
+        __syncthreads();
+        // Cloud, first slot and key count of slot g: from LDS while g is in the
+        // tile's first cloud (no dependent global loads before the search).
+        auto cloud_of = [&](uint32_t g, uint32_t& c, uint32_t& b, uint32_t& nk) {
+            if (g < e0) {
+                c = c0;
+                b = b0;
+                nk = e0 - b0;
+            } else {
+                c = seg_find_near(ubase, C, g, c0);
+                b = ubase[c];
+                nk = ubase[c + 1] - b;
+            }
+        };
 #pragma unroll 1
         for (int k = 0; k < SEG_BRT_TILE / SEG_BLOCK; ++k) {
             const uint32_t g = (uint32_t)t0 + k * SEG_BLOCK + threadIdx.x;
             if (g >= total) break;
-            const uint32_t c = seg_find_near(ubase, C, g, c0);
-            const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
+            uint32_t c, b, nk;
+            cloud_of(g, c, b, nk);
             const uint32_t i = g - b;
-            if (to_clouds) static_cast<K*>(p.out[c].unique)[i] = s_win[g - w0];
+            if (to_clouds) static_cast<K*>(p.out[c].unique)[i] = s_win[g - ws];
             if (i + 1 < nk) {
                 BrtResult r;
-                if (brt_node_near<K, SEG_BRT_LIMIT, true>(as_lds<K>(s_win), (int32_t)(w0 - (int64_t)b), nk, i, true,
-                                                          &r)) {
+                const int32_t wrel = (int32_t)(ws - (int64_t)b);
+                const bool near = c == c0
+                                      ? brt_node_near<K, SEG_BRT_LIMIT, false>(as_lds<K>(s_fast), wrel, nk, i, true, &r)
+                                      : brt_node_near<K, SEG_BRT_LIMIT, true>(as_lds<K>(s_win), wrel, nk, i, true, &r);
+                if (near) {
                     brt_store(arrays(b), i, r);
                     if (to_clouds) brt_store(cloud_arrays(c), i, r);
                 } else {
@@ -336,9 +387,10 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* _
         const uint32_t nlong = s_nlong;
         for (uint32_t q = threadIdx.x; q < nlong; q += SEG_BLOCK) {
             const uint32_t g = s_long[q];
-            const uint32_t c = seg_find_near(ubase, C, g, c0);
-            const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
-            const CodeView<K> view{uniq + b, as_lds<K>(s_win), (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0)};
+            uint32_t c, b, nk;
+            cloud_of(g, c, b, nk);
+            // The raw window holds real codes for every index of the cloud inside it.
+            const CodeView<K> view{uniq + b, as_lds<K>(s_win), (int32_t)(ws - (int64_t)b), (uint32_t)W};
             BrtResult r;
             brt_node<K>(view, nk, g - b, true, &r);
             brt_store(arrays(b), g - b, r);
@@ -361,7 +413,9 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
     __shared__ uint16_t s_meta[SEG_BRT_TILE + 2 * SEG_BRT_HALO];
     __shared__ uint32_t s_list[SEG_BRT_TILE];
     __shared__ uint32_t s_tmp[SEG_BLOCK / 64 + 2];
-    __shared__ uint32_t s_c0;
+    __shared__ uint32_t s_c0, s_b0, s_e0, s_nb0;
+    __shared__ int s_rl0;
+    __shared__ float s_min0, s_range0;
     const uint32_t C = desc[0];
     const uint32_t total = ubase[C];
     const uint32_t tiles = ceil_div_u32(total, SEG_BRT_TILE);
@@ -370,7 +424,20 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
         const int64_t w0 = t0 > SEG_BRT_HALO ? t0 - SEG_BRT_HALO : 0;
         const int64_t w1 = t0 + SEG_BRT_TILE + SEG_BRT_HALO < total ? t0 + SEG_BRT_TILE + SEG_BRT_HALO : total;
         __syncthreads();
-        if (threadIdx.x == 0) s_c0 = seg_find(ubase, C, (uint32_t)t0);
+        if (threadIdx.x < 64) {
+            // The tile's first cloud: slot range, node base, root level, quantiser.
+            // (prefix_n[s_b0] is only used when the cloud has a radix node.)
+            const uint32_t c = seg_find_wave(ubase, C, (uint32_t)t0);
+            if (threadIdx.x == 0) {
+                s_c0 = c;
+                s_b0 = ubase[c];
+                s_e0 = ubase[c + 1];
+                s_nb0 = node_base[c];
+                s_rl0 = prefix_n[s_b0] / 3;
+                s_min0 = p.min_c[c];
+                s_range0 = p.range_c[c];
+            }
+        }
         stage_links<SEG_BLOCK, SEG_BRT_TILE + 2 * SEG_BRT_HALO>(s_left, s_meta, left, ec, hll, hlr, w0, w1);
         __syncthreads();
         const uint32_t c0 = s_c0;
@@ -399,8 +466,20 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
 #pragma unroll 1
         for (uint32_t q = threadIdx.x; q < count; q += SEG_BLOCK) {
             const uint32_t g = s_list[q];
-            const uint32_t c = seg_find_near(ubase, C, g, c0);
-            const uint32_t b = ubase[c], nk = ubase[c + 1] - b;
+            // Cloud data from LDS while g is in the tile's first cloud.
+            uint32_t c = s_c0, b = s_b0, e = s_e0, nbase = s_nb0;
+            int root_level = s_rl0;
+            float mn = s_min0, rg = s_range0;
+            if (g >= e) {
+                c = seg_find_near(ubase, C, g, c0);
+                b = ubase[c];
+                e = ubase[c + 1];
+                nbase = node_base[c];
+                root_level = prefix_n[b] / 3;
+                mn = p.min_c[c];
+                rg = p.range_c[c];
+            }
+            const uint32_t nk = e - b;
             const uint32_t i = g - b;
             if (i + 1 >= nk) continue;
             const LinkView lv{left + b, hll + b, hlr + b, ec + b, as_lds<int32_t>(s_left), as_lds<uint16_t>(s_meta),
@@ -408,9 +487,8 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __rest
             // Nodes go to the cloud's own buffer when there is one (then only there).
             tc_oct_node* dst = to_clouds ? p.out[c].nodes : nodes;
             const uint32_t cap = to_clouds ? d_octree_node_bound(nk, CodeTraits<K>::BITS) : node_cap;
-            octree_build_node<K>(dst, cap, node_base[c], offsets + b, ec + b, uniq + b, prefix_n + b, lv, i, nk - 1,
-                                 prefix_n[b] / 3, link_leaves, p.min_c[c], p.range_c[c],
-                                 to_clouds ? node_base[c] : 0u);
+            octree_build_node<K>(dst, cap, nbase, offsets + b, ec + b, uniq + b, prefix_n + b, lv, i, nk - 1,
+                                 root_level, link_leaves, mn, rg, to_clouds ? nbase : 0u);
         }
     }
 }

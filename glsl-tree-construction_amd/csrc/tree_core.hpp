@@ -251,15 +251,30 @@ TC_DEV bool brt_node_near(lds_ptr<K> w, int32_t w0, uint32_t nk, uint32_t i, boo
     };
     auto thr = [&](int t) { return t >= B ? (K)0 : (K)((K)1 << (B - 1 - t)); };  // delta > t  <=>  x < thr(t)
     const K ci = code(i);
-    // i >= 1 or the sentinel below 0 (CHECK: i - 1 wraps to >= nk).
-    const int dl = (!CHECK || i >= 1) ? dlt(ci, code(i - 1)) : -1;
+    // i >= 1 or the sentinel below 0 (CHECK: slot i's own code is read instead
+    // and the result masked, so the read needs no branch).
+    int dl;
+    if constexpr (CHECK) {
+        const K cl = code(i >= 1 ? i - 1 : i);
+        dl = i >= 1 ? dlt(ci, cl) : -1;
+    } else {
+        dl = dlt(ci, code(i - 1));
+    }
     const int dr = dlt(ci, code(i + 1));  // i + 1 <= nk - 1: always a key
     const bool fwd = dr >= dl;
     auto at = [&](uint32_t x) { return fwd ? i + x : i - x; };
+    // With CHECK, a probe outside [0, nk) (i - x wraps to >= nk) reads slot i
+    // (always staged) and fails; probes inside [0, nk) lie within LIMIT of the
+    // tile, inside the window.  Branch-free: the read is not predicated.
     auto probe = [&](uint32_t x, K t) {
         const uint32_t j = at(x);
-        if (CHECK && j >= nk) return false;
-        return (K)(ci ^ code(j)) < t;
+        if constexpr (CHECK) {
+            const bool in = j < nk;
+            const K cj = code(in ? j : i);
+            return in && (K)(ci ^ cj) < t;
+        } else {
+            return (K)(ci ^ code(j)) < t;
+        }
     };
     const int dmin = fwd ? dl : dr;
     const K tmin = thr(dmin);
