@@ -68,6 +68,12 @@ struct tc_batch {
 namespace {
 
 constexpr int SEG_BLOCK = 256;
+// Workgroups per CU of the segmented tree kernels (SEG_BLOCK = one wave per
+// SIMD, so also waves per SIMD): their launch bounds ask the compiler for this
+// occupancy and their grids hold exactly this many per CU.  The octree kernel
+// needs 96 VGPRs at 5 (6 spilled); at 8 the spills make it slower.
+constexpr int SEG_BRT_PER_CU = 8;
+constexpr int SEG_OCT_PER_CU = 5;
 constexpr int SEG_ITEMS = 8;  // point loads in flight per lane in the Morton kernel
 constexpr int SEG_BRT_TILE = 1024;  // slots per workgroup step of the tree kernels
 constexpr int SEG_BRT_HALO = 512;   // LDS window halo of the octree walk
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_finalize(const uint32_t* __re
 // Unique-key slots: radix tree (and edge counts) per cloud.  The last slot of
 // each cloud has no radix node; its edge count is 0 so the node scan skips it.
 template <typename K>
-__global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* __restrict__ ubase,
+__global__ __launch_bounds__(SEG_BLOCK, SEG_BRT_PER_CU) void k_seg_brt(Plan p, const uint32_t* __restrict__ ubase,
                                                        const K* __restrict__ uniq, uint8_t* __restrict__ prefix_n,
                                                        uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr,
                                                        int32_t* __restrict__ left, int32_t* __restrict__ parent,
@@ -319,7 +325,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* _
     const bool to_clouds = (p.desc[3] & PLAN_OUT_PTRS) != 0;
     const uint32_t total = ubase[C];
     const uint32_t tiles = ceil_div_u32(total, SEG_BRT_TILE);
-    auto arrays = [&](uint32_t b) { return BrtArrays{prefix_n + b, hll + b, hlr + b, left + b, parent + b, ec + b}; };
+    const BrtArrays all{prefix_n, hll, hlr, left, parent, ec};  // concatenated, addressed at b + index
     auto cloud_arrays = [&](uint32_t c) {
         const tc_tree_ptrs& o = p.out[c];
         return BrtArrays{o.prefix_n, o.has_leaf_left, o.has_leaf_right, o.left_child, o.parent, o.edge_count};
@@ -369,12 +375,11 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* _
             if (to_clouds) static_cast<K*>(p.out[c].unique)[i] = s_win[g - ws];
             if (i + 1 < nk) {
                 BrtResult r;
-                const int32_t wrel = (int32_t)(ws - (int64_t)b);
-                const bool near = c == c0
-                                      ? brt_node_near<K, SEG_BRT_LIMIT, false>(as_lds<K>(s_fast), wrel, nk, i, true, &r)
-                                      : brt_node_near<K, SEG_BRT_LIMIT, true>(as_lds<K>(s_win), wrel, nk, i, true, &r);
-                if (near) {
-                    brt_store(arrays(b), i, r);
+                // Nodes of later clouds in the tile (tiles that straddle a
+                // cloud boundary) take the full search below.
+                if (c == c0 && brt_node_near<K, SEG_BRT_LIMIT, false>(as_lds<K>(s_fast), (int32_t)(ws - (int64_t)b),
+                                                                       nk, i, true, &r)) {
+                    brt_store(all, i, r, b);
                     if (to_clouds) brt_store(cloud_arrays(c), i, r);
                 } else {
                     s_long[atomicAdd(&s_nlong, 1u)] = g;
@@ -393,14 +398,14 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_brt(Plan p, const uint32_t* _
             const CodeView<K> view{uniq + b, as_lds<K>(s_win), (int32_t)(ws - (int64_t)b), (uint32_t)W};
             BrtResult r;
             brt_node<K>(view, nk, g - b, true, &r);
-            brt_store(arrays(b), g - b, r);
+            brt_store(all, g - b, r, b);
             if (to_clouds) brt_store(cloud_arrays(c), g - b, r);
         }
     }
 }
 
 template <typename K>
-__global__ __launch_bounds__(SEG_BLOCK) void k_seg_octree(const uint32_t* __restrict__ desc, const uint32_t* __restrict__ ubase,
+__global__ __launch_bounds__(SEG_BLOCK, SEG_OCT_PER_CU) void k_seg_octree(const uint32_t* __restrict__ desc, const uint32_t* __restrict__ ubase,
                                                           const uint32_t* __restrict__ node_base, tc_oct_node* __restrict__ nodes,
                                                           uint32_t node_cap, const uint32_t* __restrict__ offsets,
                                                           const uint32_t* __restrict__ ec, const K* __restrict__ uniq,
@@ -566,7 +571,9 @@ int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s
     const uint32_t fgrid = ceil_div_u32(C + 1, SEG_BLOCK);
     hipLaunchKernelGGL(k_seg_finalize, dim3(fgrid), dim3(SEG_BLOCK), 0, s, p.desc, p.pt_off, b->ubase, b->totals);
     TC_LAUNCH_CHECK();
-    const uint32_t tgrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK);
+    // The u64 radix-tree kernel's two LDS windows (24 KiB) fit 6 per CU.
+    const int brt_per_cu = sizeof(K) == 4 ? SEG_BRT_PER_CU : 6;
+    const uint32_t tgrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK, brt_per_cu);
     hipLaunchKernelGGL(k_seg_brt<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p, b->ubase, uniq, b->prefix_n,
                        b->hll, b->hlr, b->left, b->parent, b->ec);
     TC_LAUNCH_CHECK();
@@ -576,7 +583,8 @@ int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s
                        b->totals + 1);
     TC_LAUNCH_CHECK();
     const uint32_t ncap = (uint32_t)(b->node_cap < 0xFFFFFFFFull ? b->node_cap : 0xFFFFFFFFull);
-    hipLaunchKernelGGL(k_seg_octree<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, b->node_base,
+    const uint32_t ogrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK, SEG_OCT_PER_CU);
+    hipLaunchKernelGGL(k_seg_octree<K>, dim3(ogrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, b->node_base,
                        b->nodes, ncap, b->offsets, b->ec, uniq, b->prefix_n, b->left, b->hll, b->hlr, link_leaves, p);
     TC_LAUNCH_CHECK();
     return TC_OK;

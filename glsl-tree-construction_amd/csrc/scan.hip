@@ -395,9 +395,9 @@ void launch(uint32_t grid, hipStream_t s, const ScanArgs& a) {
 
 size_t tc_scan_scratch_bytes(uint32_t n) { return ((size_t)ceil_div_u32(n ? n : 1, SCAN_TILE) + 1) * 8; }
 
-int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block) {
+int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu) {
     uint32_t blocks = ceil_div_u32(work_items ? work_items : 1, (uint32_t)block);
-    const uint32_t cap = (uint32_t)ctx->num_cus * 8;
+    const uint32_t cap = (uint32_t)ctx->num_cus * (uint32_t)per_cu;
     return (int)(blocks < cap ? blocks : cap);
 }
 

@@ -190,7 +190,11 @@ int tc_fill_u32(tc_ctx* ctx, uint32_t* ptr, uint32_t value, size_t count, hipStr
 int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0, void* out1,
                    uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, const uint32_t* seg_off,
                    uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s);
-int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block);
+// Grid of a kernel that loops over its work (grid-stride): one workgroup per
+// `block` items, at most per_cu per CU -- the number that is resident at once
+// (a workgroup that does not fit would start only after others finished all
+// their work, a tail as long as the whole kernel).
+int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu = 8);
 
 // ---------------------------------------------------------------- device side
 #ifdef __HIP_DEVICE_COMPILE__

@@ -159,18 +159,20 @@ struct BrtArrays {
 // Store node i: its own entries, the parent links of its internal children,
 // and (with ec) their edge counts; node 0 also writes the root's parent slot
 // (the reference's zero-initialised entry, src/main.rs:103) and count, 1.
-TC_DEV void brt_store(const BrtArrays& o, uint32_t i, const BrtResult& r) {
-    o.prefix_n[i] = (uint8_t)r.dnode;
-    o.left_child[i] = (int32_t)r.gamma;
-    o.hll[i] = r.leaf_l;
-    o.hlr[i] = r.leaf_r;
-    if (!r.leaf_l) o.parent[r.gamma] = (int32_t)i;
-    if (!r.leaf_r) o.parent[r.gamma + 1] = (int32_t)i;
-    if (i == 0) o.parent[0] = 0;
+// Node i's results (cloud-local indices and values) stored at slot base + index
+// of the arrays (base: the cloud's first slot in concatenated arrays, else 0).
+TC_DEV void brt_store(const BrtArrays& o, uint32_t i, const BrtResult& r, uint32_t base = 0) {
+    o.prefix_n[base + i] = (uint8_t)r.dnode;
+    o.left_child[base + i] = (int32_t)r.gamma;
+    o.hll[base + i] = r.leaf_l;
+    o.hlr[base + i] = r.leaf_r;
+    if (!r.leaf_l) o.parent[base + r.gamma] = (int32_t)i;
+    if (!r.leaf_r) o.parent[base + r.gamma + 1] = (int32_t)i;
+    if (i == 0) o.parent[base] = 0;
     if (o.ec) {
-        if (!r.leaf_l) o.ec[r.gamma] = r.ec_l;
-        if (!r.leaf_r) o.ec[r.gamma + 1] = r.ec_r;
-        if (i == 0) o.ec[0] = 1u;
+        if (!r.leaf_l) o.ec[base + r.gamma] = r.ec_l;
+        if (!r.leaf_r) o.ec[base + r.gamma + 1] = r.ec_r;
+        if (i == 0) o.ec[base] = 1u;
     }
 }
 
