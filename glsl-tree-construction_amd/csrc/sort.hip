@@ -245,11 +245,15 @@ int dispatch_pass(tc_ctx* ctx, SortShape sh, uint32_t tiles, hipStream_t s, cons
     return TC_ERR_INVALID;
 }
 
-// Eight workgroups (32 waves) per CU: enough loads in flight to stream at
-// HBM speed; the merge's atomic chains stay gridDim / HIST_COPIES long.
+// As many workgroups as are resident at once: eight per CU (32 waves, enough
+// loads in flight to stream at HBM speed) for u32 keys; five for u64 keys,
+// whose 32 KiB of LDS sub-histograms fit five per CU (a grid-stride workgroup
+// that does not fit would start only after the others finished).  The merge's
+// atomic chains stay gridDim / HIST_COPIES long.
+template <typename K>
 uint32_t hist_grid(tc_ctx* ctx, uint32_t n) {
     uint32_t blocks = ceil_div_u32(n, HIST_BLOCK * 16);
-    uint32_t cap = (uint32_t)ctx->num_cus * 8;
+    uint32_t cap = (uint32_t)ctx->num_cus * (sizeof(K) == 4 ? 8u : 5u);
     if (blocks < 1) blocks = 1;
     return blocks < cap ? blocks : cap;
 }
@@ -280,7 +284,7 @@ template <typename K>
 int histogram_impl(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist_out, uint32_t* zero_ptr,
                    uint32_t zero_words, hipStream_t s) {
     ProfScope prof(ctx, s, K_HIST);
-    hipLaunchKernelGGL((k_global_histogram<K>), dim3(hist_grid(ctx, n)), dim3(HIST_BLOCK), 0, s, keys, n,
+    hipLaunchKernelGGL((k_global_histogram<K>), dim3(hist_grid<K>(ctx, n)), dim3(HIST_BLOCK), 0, s, keys, n,
                        ctx->hist_acc, zero_ptr, zero_words);
     TC_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_hist_finish, dim3(sizeof(K)), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, hist_out);
@@ -309,7 +313,7 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
     // The histogram kernel also clears the tile counters and look-back region 0.
     if (pts) {
         ProfScope prof(ctx, s, K_MORTON);
-        hipLaunchKernelGGL((k_morton_hist<K>), dim3(hist_grid(ctx, n)), dim3(HIST_BLOCK), 0, s,
+        hipLaunchKernelGGL((k_morton_hist<K>), dim3(hist_grid<K>(ctx, n)), dim3(HIST_BLOCK), 0, s,
                            reinterpret_cast<const float4*>(pts), keys, vals, n, min_coord, range, ctx->hist_acc,
                            ss.ctr, 64 + region_words);
         TC_LAUNCH_CHECK();
