@@ -112,51 +112,6 @@ Plan plan_at(T* base, uint32_t C) {
 
 Plan plan_view(const tc_batch* b) { return plan_at(b->plan, b->cap_clouds); }
 
-// Largest c in [lo, n-1] with off[c] <= x (the non-empty segment holding x),
-// given off[lo] <= x.
-TC_DEV uint32_t seg_search(const uint32_t* off, uint32_t lo, uint32_t n, uint32_t x) {
-    uint32_t hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= x) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-TC_DEV uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t x) { return seg_search(off, 0, n, x); }
-
-// seg_find by a whole wave (all 64 lanes active, same x): each round tests 64
-// evenly spaced candidates at once, so 1,024 segments take 2 dependent loads
-// instead of 10.  off[c] <= x holds for a prefix of c (off is nondecreasing),
-// so the ballot is a run of low lanes; the last of them brackets the answer.
-TC_DEV uint32_t seg_find_wave(const uint32_t* off, uint32_t n, uint32_t x) {
-    uint32_t lo = 0, hi = n;  // answer in [lo, hi), off[lo] <= x
-    const uint32_t lane = __lane_id();
-    while (hi - lo > 1) {
-        const uint32_t step = (hi - lo + 63) / 64;
-        const uint32_t k = lo + lane * step;
-        const bool le = k < hi && off[k] <= x;
-        const uint64_t m = __ballot(le);
-        const uint32_t last = 63u - (uint32_t)__builtin_clzll(m);  // lane 0 always holds: off[lo] <= x
-        lo = lo + last * step;
-        hi = lo + step < hi ? lo + step : hi;
-    }
-    return lo;
-}
-
-// As seg_find, starting from c0, the segment of an index <= x (a tile's first
-// slot): a few steps forward cover the usual case of a tile inside one or two
-// clouds, a binary search the rest.
-TC_DEV uint32_t seg_find_near(const uint32_t* off, uint32_t n, uint32_t x, uint32_t c0) {
-    uint32_t c = c0;
-    for (int k = 0; k < 4; ++k) {
-        if (c + 1 < n && off[c + 1] <= x) ++c;
-        else return c;
-    }
-    return seg_search(off, c, n, x);
-}
-
 // Morton codes + per-cloud digit histograms, one sort tile at a time (the
 // histogram is fused into the producer, so the codes are read only by the
 // binning passes).  Blocks take contiguous runs of tiles and flush their LDS
@@ -176,7 +131,12 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
     for (uint32_t i = tid; i < DIG * 256; i += SEG_BLOCK) s_h[i] = 0;
     uint32_t cur = 0xFFFFFFFFu;
     for (uint32_t t = t0; t < t1; ++t) {
-        if (tid == 0) s_cloud = seg_find(p.first_tile, C, t);
+        if (tid < 64) {
+            // Tiles run in order: after the first, the cloud is usually the same one.
+            const uint32_t c = cur == 0xFFFFFFFFu ? seg_find_wave(p.first_tile, C, t)
+                                                  : seg_find_near(p.first_tile, C, t, cur);
+            if (tid == 0) s_cloud = c;
+        }
         __syncthreads();
         const uint32_t c = s_cloud;
         if (c != cur) {
@@ -260,7 +220,10 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(Plan p, const K* __restr
     // Grid = tile capacity, tile = workgroup id (see k_onesweep).
     const uint32_t tile = blockIdx.x;
     if (tile >= T) return;
-    if (threadIdx.x == 0) sm.aux = seg_find(p.first_tile, C, tile);
+    if (threadIdx.x < 64) {
+        const uint32_t c = seg_find_wave(p.first_tile, C, tile);
+        if (threadIdx.x == 0) sm.aux = c;
+    }
     __syncthreads();
     const uint32_t c = sm.aux;
     const uint32_t first = p.first_tile[c];

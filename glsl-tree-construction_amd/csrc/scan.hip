@@ -183,6 +183,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     __shared__ uint32_t s_tmp[SCAN_BLOCK / 64 + 2];
     __shared__ uint32_t s_excl;
     __shared__ uint32_t s_fallback;
+    __shared__ uint32_t s_seg0;  // segmented modes: the segment of the tile's first element
     __shared__ __attribute__((aligned(16))) T s_items[SCAN_TILE];
 
     const uint32_t tid = threadIdx.x;
@@ -216,12 +217,19 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     uint32_t start_mask = 0;  // bit j: element i0 + j starts a segment
     uint32_t seg_first = 0;
 
-    if (MODE != MODE_ECSCAN) {
+    // Wave 0 finds the tile's first segment (64 candidates per load round)
+    // while the other waves load the tile; each thread then steps forward
+    // from it instead of searching all segments itself.
+    if (is_seg(MODE) && wave == 0 && nseg > 0 && base < n) {
+        const uint32_t c = seg_find_wave(a.seg_off, nseg, base);
+        if (lane_id() == 0) s_seg0 = c;
+    }
+    if (MODE != MODE_ECSCAN) {  // (segmented modes always take this sync)
         load_tile<T>(static_cast<const T*>(a.in0), base, cnt, s_items, aligned);
         __syncthreads();
     }
     if (is_seg(MODE) && nseg > 0 && i0 < n) {
-        uint32_t seg = find_seg(a.seg_off, nseg, i0);
+        uint32_t seg = seg_find_near(a.seg_off, nseg, i0, s_seg0);
         seg_first = seg;
         uint32_t seg_start = a.seg_off[seg], seg_next = a.seg_off[seg + 1];
 #pragma unroll

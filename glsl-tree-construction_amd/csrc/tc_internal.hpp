@@ -205,6 +205,54 @@ int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu =
 
 TC_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
+// Segment lookups over a nondecreasing offset array off[0..n] (off[0] = 0):
+// the segment of index x is the largest c in [0, n-1] with off[c] <= x, which
+// skips empty segments.
+// Largest c in [lo, n-1] with off[c] <= x (the non-empty segment holding x),
+// given off[lo] <= x.
+TC_DEV uint32_t seg_search(const uint32_t* off, uint32_t lo, uint32_t n, uint32_t x) {
+    uint32_t hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+TC_DEV uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t x) { return seg_search(off, 0, n, x); }
+
+// seg_find by a whole wave (all 64 lanes active, same x): each round tests 64
+// evenly spaced candidates at once, so 1,024 segments take 2 dependent loads
+// instead of 10.  off[c] <= x holds for a prefix of c (off is nondecreasing),
+// so the ballot is a run of low lanes; the last of them brackets the answer.
+TC_DEV uint32_t seg_find_wave(const uint32_t* off, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;  // answer in [lo, hi), off[lo] <= x
+    const uint32_t lane = __lane_id();
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t k = lo + lane * step;
+        const bool le = k < hi && off[k] <= x;
+        const uint64_t m = __ballot(le);
+        const uint32_t last = 63u - (uint32_t)__builtin_clzll(m);  // lane 0 always holds: off[lo] <= x
+        lo = lo + last * step;
+        hi = lo + step < hi ? lo + step : hi;
+    }
+    return lo;
+}
+
+// As seg_find, starting from c0, the segment of an index <= x (a tile's first
+// slot): a few steps forward cover the usual case of a tile inside one or two
+// clouds, a binary search the rest.
+TC_DEV uint32_t seg_find_near(const uint32_t* off, uint32_t n, uint32_t x, uint32_t c0) {
+    uint32_t c = c0;
+    for (int k = 0; k < 4; ++k) {
+        if (c + 1 < n && off[c + 1] <= x) ++c;
+        else return c;
+    }
+    return seg_search(off, c, n, x);
+}
+
 // tc_octree_node_bound on the device: 1 + sum over levels of min(8^L, n/2).
 TC_DEV uint32_t d_octree_node_bound(uint32_t n_keys, int code_bits) {
     if (n_keys < 2) return 0;
