@@ -287,8 +287,8 @@ def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04, plain=True):
     }
 
 
-def bench_served(ctx, rates=(1000, 2000, 5000, 10000, 12000, 14000, 16000, 20000, 50000), pool=256, max_req=2000,
-                 window_s=0.5, slo_ms=10.0, seed=0xC0FFEE05):
+def bench_served(ctx, rates=(1000, 2000, 5000, 7500, 10000, 12000, 14000, 16000, 18000, 20000, 50000), pool=256,
+                 max_req=2000, window_s=0.5, slo_ms=10.0, seed=0xC0FFEE05):
     """Poisson arrivals of build requests (sizes log-uniform in [2^12, 2^22],
     uniform xyz in [0,1024)^3) against the server (64 GiB budget, 2 streams) and
     against a one-request-at-a-time plain loop, on the same traces.  Inputs are
