@@ -71,7 +71,7 @@ constexpr int SEG_BLOCK = 256;
 // Workgroups per CU of the segmented tree kernels (SEG_BLOCK = one wave per
 // SIMD, so also waves per SIMD): their launch bounds ask the compiler for this
 // occupancy and their grids hold exactly this many per CU.  The octree kernel
-// needs 96 VGPRs at 5 (6 spilled); at 8 the spills make it slower.
+// fits 5 without spills (96 VGPRs); at 8 (27 registers spilled) it is slower.
 constexpr int SEG_BRT_PER_CU = 8;
 constexpr int SEG_OCT_PER_CU = 5;
 constexpr int SEG_ITEMS = 8;  // point loads in flight per lane in the Morton kernel
@@ -450,13 +450,13 @@ __global__ __launch_bounds__(SEG_BLOCK, SEG_OCT_PER_CU) void k_seg_octree(const 
             const uint32_t nk = e - b;
             const uint32_t i = g - b;
             if (i + 1 >= nk) continue;
-            const LinkView lv{left + b, hll + b, hlr + b, ec + b, as_lds<int32_t>(s_left), as_lds<uint16_t>(s_meta),
-                              (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0)};
+            const LinkView lv{left, hll, hlr, ec, as_lds<int32_t>(s_left), as_lds<uint16_t>(s_meta),
+                              (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0), b};
             // Nodes go to the cloud's own buffer when there is one (then only there).
             tc_oct_node* dst = to_clouds ? p.out[c].nodes : nodes;
             const uint32_t cap = to_clouds ? d_octree_node_bound(nk, CodeTraits<K>::BITS) : node_cap;
-            octree_build_node<K>(dst, cap, nbase, offsets + b, ec + b, uniq + b, prefix_n + b, lv, i, nk - 1,
-                                 root_level, link_leaves, mn, rg, to_clouds ? nbase : 0u);
+            octree_build_node<K>(dst, cap, nbase, offsets, ec, uniq, prefix_n, lv, i, nk - 1, root_level,
+                                 link_leaves, mn, rg, to_clouds ? nbase : 0u);
         }
     }
 }

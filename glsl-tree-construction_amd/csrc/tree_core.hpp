@@ -38,7 +38,9 @@ TC_DEV lds_ptr<K> as_lds(const K* p) {
 // Radix-tree links of one cloud (cloud-local indices), read through an
 // optional LDS copy of the window [w0, w0 + wlen): the left child and a packed
 // word edge_count | has_leaf_left << 8 | has_leaf_right << 9 (edge counts are
-// at most 21).  The octree's walks down the radix tree read these.
+// at most 21).  The octree's walks down the radix tree read these.  Outside
+// the window, local index x is read at gb + x of the arrays (gb: the cloud's
+// first slot in concatenated arrays, else 0; the pointers stay uniform).
 struct LinkView {
     const int32_t* __restrict__ left;
     const uint8_t* __restrict__ hll;
@@ -48,6 +50,7 @@ struct LinkView {
     lds_ptr<uint16_t> wmeta = nullptr;
     int32_t w0 = 0;
     uint32_t wlen = 0;
+    uint32_t gb = 0;
 
     static TC_DEV uint16_t pack(uint32_t e, uint8_t l, uint8_t r) {
         return (uint16_t)((e < 255u ? e : 255u) | ((uint32_t)(l != 0) << 8) | ((uint32_t)(r != 0) << 9));
@@ -55,12 +58,12 @@ struct LinkView {
     TC_DEV int32_t left_child(uint32_t x) const {
         const uint32_t r = x - (uint32_t)w0;
         if (r < wlen) return wleft[r];
-        return left[x];
+        return left[gb + x];
     }
     TC_DEV uint32_t meta(uint32_t x) const {
         const uint32_t r = x - (uint32_t)w0;
         if (r < wlen) return wmeta[r];
-        return pack(ec[x], hll[x], hlr[x]);
+        return pack(ec[gb + x], hll[gb + x], hlr[gb + x]);
     }
 };
 
@@ -335,6 +338,8 @@ TC_DEV uint32_t child_slot(K code, int level) {
 // The walk replaces the upward ancestor search of shader/octree.comp:157-165
 // and the racy mask updates (each node is written once, whole, by one lane).
 // offsets[] hold global node indices; stored links are local (minus node_base).
+// Indices a, c are cloud-local; arrays are read at gb + index (lv.gb, see
+// LinkView).
 template <typename K>
 TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap, uint32_t node_base,
                               const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ec,
@@ -342,11 +347,12 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
                               const LinkView& lv, uint32_t a, uint32_t nb, int root_level, int link_leaves,
                               float min_coord, float range, uint32_t write_base = 0) {
     constexpr int BITS = CodeTraits<K>::BITS;
-    const uint32_t cnt = ec[a];
+    const uint32_t gb = lv.gb;
+    const uint32_t cnt = ec[gb + a];
     if (cnt == 0) return;
-    const uint32_t off = offsets[a];
-    const K code = codes[a];
-    const int bottom = prefix_n[a] / 3;
+    const uint32_t off = offsets[gb + a];
+    const K code = codes[gb + a];
+    const int bottom = prefix_n[gb + a] / 3;
 
     // Walk (LDS reads only while inside the window).  kind: 0 none,
     // 1 radix leaf, 2 internal node with its own chain.
@@ -400,10 +406,10 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
     for (int p = 0; p < 8; ++p) {
         if (kind[p] == 0) continue;
         const uint32_t c = cand[p];
-        const uint32_t slot = child_slot<K>(codes[c], bottom + 1);
+        const uint32_t slot = child_slot<K>(codes[gb + c], bottom + 1);
         int32_t v = (int32_t)c;
         if (kind[p] == 2) {
-            v = (int32_t)(offsets[c] + (lv.meta(c) & 255u) - 1 - node_base);
+            v = (int32_t)(offsets[gb + c] + (lv.meta(c) & 255u) - 1 - node_base);
             bmask |= 1 << slot;
         }
 #pragma unroll
