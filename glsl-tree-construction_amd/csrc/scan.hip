@@ -177,13 +177,8 @@ __global__ void k_scan_prep(ScanCtl* ctl) {
     ctl->seq = (ctl->seq + 1u) & 0x3FFFFFFFu;
 }
 
-// Plain u32 unique asks for 8 waves per SIMD (4 workgroups per CU; 64 VGPRs
-// with 2 spilled, from 65): 0.140 -> 0.134 ms on 2^26 codes.  The segmented
-// form would spill 11 registers for it and gets slower, so it keeps 3.
-constexpr int scan_min_waves(int m) { return m == MODE_UNIQUE32 ? 8 : 1; }
-
 template <int MODE>
-__global__ __launch_bounds__(SCAN_BLOCK, scan_min_waves(MODE)) void k_scan_tiles(ScanArgs a) {
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     using T = ItemT<MODE>;
     __shared__ uint32_t s_tmp[SCAN_BLOCK / 64 + 2];
     __shared__ uint32_t s_excl;
@@ -217,8 +212,15 @@ __global__ __launch_bounds__(SCAN_BLOCK, scan_min_waves(MODE)) void k_scan_tiles
     const uint32_t cnt = n > base ? (n - base < SCAN_TILE ? n - base : SCAN_TILE) : 0u;
     const uint32_t r0 = tid * SCAN_IPT;  // tile-relative index of this thread's first element
     const uint32_t i0 = base + r0;
-    uint32_t val[SCAN_IPT];
-    T item[is_unique(MODE) ? SCAN_IPT : 1];
+    // Per-element values; in the unique modes they are 0/1 run-start flags,
+    // kept as bits of one register instead (V(j) reads either form).
+    static_assert(SCAN_IPT <= 32, "flags of one thread's elements fit one word");
+    uint32_t val[is_unique(MODE) ? 1 : SCAN_IPT];
+    uint32_t keep = 0;
+    auto V = [&](int j) -> uint32_t {
+        if constexpr (is_unique(MODE)) return (keep >> j) & 1u;
+        else return val[j];
+    };
     uint32_t start_mask = 0;  // bit j: element i0 + j starts a segment
     uint32_t seg_first = 0;
 
@@ -261,8 +263,7 @@ __global__ __launch_bounds__(SCAN_BLOCK, scan_min_waves(MODE)) void k_scan_tiles
             const uint32_t r = r0 + j;
             const T k = r < cnt ? s_items[r] : (T)0;
             const bool starts = (base + r == 0) || ((start_mask >> j) & 1u);
-            item[j] = k;
-            val[j] = (r < cnt && (starts || k != prev)) ? 1u : 0u;
+            keep |= ((r < cnt && (starts || k != prev)) ? 1u : 0u) << j;
             prev = k;
         }
     } else {  // MODE_ECSCAN: edge count of radix node i, root entry 1
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(SCAN_BLOCK, scan_min_waves(MODE)) void k_scan_tiles
 
     uint32_t tsum = 0;
 #pragma unroll
-    for (int j = 0; j < SCAN_IPT; ++j) tsum += val[j];
+    for (int j = 0; j < SCAN_IPT; ++j) tsum += V(j);
     uint32_t block_total;
     const uint32_t texcl = block_exclusive_scan<SCAN_BLOCK>(tsum, s_tmp, &block_total);
 
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(SCAN_BLOCK, scan_min_waves(MODE)) void k_scan_tiles
                 while (seg + 1 < nseg && a.seg_off[seg + 1] <= i0 + j) ++seg;
                 a.seg_out[seg] = run;
             }
-            run += val[j];
+            run += V(j);
         }
     }
 
@@ -345,19 +346,25 @@ __global__ __launch_bounds__(SCAN_BLOCK, scan_min_waves(MODE)) void k_scan_tiles
 #pragma unroll
         for (int j = 0; j < SCAN_IPT; ++j) {
             s_out[r0 + j] = run;
-            run += val[j];
+            run += V(j);
         }
         __syncthreads();
         uint32_t* out = static_cast<uint32_t*>(a.out0);
         for (uint32_t k = tid; k < cnt; k += SCAN_BLOCK) out[base + k] = s_out[k];
     } else if (is_unique(MODE)) {
         // Compact the tile's kept keys in LDS, then store them contiguously.
+        // The keys are read back from the LDS copy of the tile only now (it is
+        // unchanged since the load), so they are not held in registers through
+        // the scan and the look-back.  Slots past cnt are read but not kept.
+        T item[SCAN_IPT];
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; ++j) item[j] = s_items[r0 + j];
         uint32_t run = texcl;
         __syncthreads();  // everyone has read its items
 #pragma unroll
         for (int j = 0; j < SCAN_IPT; ++j) {
-            if (val[j]) s_items[run] = item[j];
-            run += val[j];
+            if (V(j)) s_items[run] = item[j];
+            run += V(j);
         }
         __syncthreads();
         T* out = static_cast<T*>(a.out0);
@@ -370,8 +377,8 @@ __global__ __launch_bounds__(SCAN_BLOCK, scan_min_waves(MODE)) void k_scan_tiles
             __syncthreads();  // the keys are out of LDS
 #pragma unroll
             for (int j = 0; j < SCAN_IPT; ++j) {
-                if (val[j]) s_pos[run] = i0 + j;
-                run += val[j];
+                if (V(j)) s_pos[run] = i0 + j;
+                run += V(j);
             }
             __syncthreads();
             uint32_t* first = static_cast<uint32_t*>(a.out1);
@@ -384,10 +391,10 @@ __global__ __launch_bounds__(SCAN_BLOCK, scan_min_waves(MODE)) void k_scan_tiles
 #pragma unroll
         for (int j = 0; j < SCAN_IPT; ++j) {
             if (i0 + j < n) {
-                ec[i0 + j] = val[j];
+                ec[i0 + j] = V(j);
                 off[i0 + j] = run;
             }
-            run += val[j];
+            run += V(j);
         }
     }
     if (tile == num_tiles - 1 && tid == SCAN_BLOCK - 1) {
