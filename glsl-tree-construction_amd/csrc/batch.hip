@@ -166,7 +166,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
 #pragma unroll
             for (int u = 0; u < SEG_ITEMS; ++u) {
                 const uint32_t i = i0 + u * SEG_BLOCK;
-                if (i < end) qv[u] = src[i - c0];
+                if (i < end) qv[u] = ld_stream(&src[i - c0]);
             }
 #pragma unroll
             for (int u = 0; u < SEG_ITEMS; ++u) {

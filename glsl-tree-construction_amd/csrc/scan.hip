@@ -156,8 +156,10 @@ TC_DEV uint32_t elem_value(const ScanArgs& a, uint32_t n, uint32_t i, uint32_t n
     return i == 0 ? 1u : (uint32_t)(pn[i] / 3 - pn[p] / 3);
 }
 
-// Coalesced tile load into LDS (tile-relative positions).
-template <typename T>
+// Coalesced tile load into LDS (tile-relative positions).  STREAM: the input
+// is read only here (unique's sorted keys), so full tiles load it
+// non-temporally (ld_stream); the scans' inputs are read again later.
+template <typename T, bool STREAM>
 TC_DEV void load_tile(const T* __restrict__ in, uint32_t base, uint32_t cnt, T* s, bool aligned) {
     constexpr int VEC = 16 / sizeof(T);
     if (aligned && cnt == SCAN_TILE) {
@@ -166,7 +168,8 @@ TC_DEV void load_tile(const T* __restrict__ in, uint32_t base, uint32_t cnt, T* 
         V* vs = reinterpret_cast<V*>(s);
 #pragma unroll
         for (int r = 0; r < (int)(SCAN_TILE / VEC / SCAN_BLOCK); ++r)
-            vs[r * SCAN_BLOCK + threadIdx.x] = vin[r * SCAN_BLOCK + threadIdx.x];
+            vs[r * SCAN_BLOCK + threadIdx.x] =
+                STREAM ? ld_stream(&vin[r * SCAN_BLOCK + threadIdx.x]) : vin[r * SCAN_BLOCK + threadIdx.x];
     } else {
         for (uint32_t k = threadIdx.x; k < cnt; k += SCAN_BLOCK) s[k] = in[base + k];
     }
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         if (lane_id() == 0) s_seg0 = c;
     }
     if (MODE != MODE_ECSCAN) {  // (segmented modes always take this sync)
-        load_tile<T>(static_cast<const T*>(a.in0), base, cnt, s_items, aligned);
+        load_tile<T, is_unique(MODE)>(static_cast<const T*>(a.in0), base, cnt, s_items, aligned);
         __syncthreads();
     }
     if (is_seg(MODE) && nseg > 0 && i0 < n) {

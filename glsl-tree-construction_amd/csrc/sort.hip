@@ -107,7 +107,7 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
     for (uint32_t c = blockIdx.x; c < full; c += gridDim.x) {
         typename KeyVec<K>::V v[HG_ITEMS];
 #pragma unroll
-        for (int u = 0; u < HG_ITEMS; ++u) v[u] = kv[c * CHUNK + u * HIST_BLOCK + tid];
+        for (int u = 0; u < HG_ITEMS; ++u) v[u] = ld_stream(&kv[c * CHUNK + u * HIST_BLOCK + tid]);
 #pragma unroll
         for (int u = 0; u < HG_ITEMS; ++u) {
             K k[VN];
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_morton_hist(
         const uint32_t b0 = c * CHUNK + tid;
         float4 p[MH_ITEMS];
 #pragma unroll
-        for (int u = 0; u < MH_ITEMS; ++u) p[u] = pts[b0 + u * HIST_BLOCK];
+        for (int u = 0; u < MH_ITEMS; ++u) p[u] = ld_stream(&pts[b0 + u * HIST_BLOCK]);
 #pragma unroll
         for (int u = 0; u < MH_ITEMS; ++u) {
             const K code = encode_point<K>(p[u], min_coord, range);

@@ -134,10 +134,10 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
     uint32_t pos[KPT];
     if constexpr (FULL) {
 #pragma unroll
-        for (int i = 0; i < KPT; ++i) keys[i] = sk[base + wofs + i * 64];
+        for (int i = 0; i < KPT; ++i) keys[i] = ld_stream(&sk[base + wofs + i * 64]);
         if (PAIRS) {
 #pragma unroll
-            for (int i = 0; i < KPT; ++i) vals[i] = sv[base + wofs + i * 64];
+            for (int i = 0; i < KPT; ++i) vals[i] = ld_stream(&sv[base + wofs + i * 64]);
         }
     } else {
 #pragma unroll

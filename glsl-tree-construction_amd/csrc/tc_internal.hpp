@@ -205,6 +205,31 @@ int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu =
 
 TC_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
+// Loads of data that is read exactly once (the points, each sort pass's source
+// keys and payload, unique's sorted input): non-temporal, so they do not
+// displace data that the next kernel reads again -- above all the last pass's
+// output, which the MALL can hold (measured at 2^26: Morton + histogram
+// 0.289 -> 0.257 ms, binning pass 0.154 -> 0.147 ms).
+template <typename T>
+TC_DEV T ld_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+TC_DEV float4 ld_stream(const float4* p) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+TC_DEV uint4 ld_stream(const uint4* p) {
+    typedef unsigned int v4 __attribute__((ext_vector_type(4)));
+    const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+TC_DEV ulonglong2 ld_stream(const ulonglong2* p) {
+    typedef unsigned long long v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return make_ulonglong2(v.x, v.y);
+}
+
 // Segment lookups over a nondecreasing offset array off[0..n] (off[0] = 0):
 // the segment of index x is the largest c in [0, n-1] with off[c] <= x, which
 // skips empty segments.
