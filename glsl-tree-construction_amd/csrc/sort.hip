@@ -202,7 +202,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(
     if (tile >= num_tiles) return;
     const uint32_t base = tile * Sh::TILE;
     const uint32_t cnt = n - base < Sh::TILE ? n - base : Sh::TILE;
-    onesweep_tile<K, PAIRS, BLOCK, KPT, RB>(sm, sk, dk, sv, dv, base, cnt, shift, tile, 0u, 0u, digit_prefix, 0u,
+    onesweep_tile<K, PAIRS, BLOCK, KPT, RB, !PAIRS>(sm, sk, dk, sv, dv, base, cnt, shift, tile, 0u, 0u, digit_prefix, 0u,
                                             lb_cur, lb_next, spin_limit);
 }
 

@@ -205,11 +205,10 @@ int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu =
 
 TC_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
-// Loads of data that is read exactly once (the points, each sort pass's source
-// keys and payload, unique's sorted input): non-temporal, so they do not
-// displace data that the next kernel reads again -- above all the last pass's
-// output, which the MALL can hold (measured at 2^26: Morton + histogram
-// 0.289 -> 0.257 ms, binning pass 0.154 -> 0.147 ms).
+// Loads of data that is read exactly once (the points, a plain key sort's
+// source keys, unique's sorted input): non-temporal (measured at 2^26:
+// Morton + histogram 0.289 -> 0.257 ms, binning pass 0.154 -> 0.147 ms; the
+// HBM byte counts of the pass did not change).
 template <typename T>
 TC_DEV T ld_stream(const T* p) {
     return __builtin_nontemporal_load(p);
