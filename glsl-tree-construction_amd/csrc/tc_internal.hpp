@@ -232,8 +232,8 @@ TC_DEV ulonglong2 ld_stream(const ulonglong2* p) {
 // Segment lookups over a nondecreasing offset array off[0..n] (off[0] = 0):
 // the segment of index x is the largest c in [0, n-1] with off[c] <= x, which
 // skips empty segments.
-// Largest c in [lo, n-1] with off[c] <= x (the non-empty segment holding x),
-// given off[lo] <= x.
+//
+// Binary search for it in [lo, n-1], given off[lo] <= x.
 TC_DEV uint32_t seg_search(const uint32_t* off, uint32_t lo, uint32_t n, uint32_t x) {
     uint32_t hi = n;
     while (hi - lo > 1) {
@@ -244,11 +244,9 @@ TC_DEV uint32_t seg_search(const uint32_t* off, uint32_t lo, uint32_t n, uint32_
     return lo;
 }
 
-TC_DEV uint32_t seg_find(const uint32_t* off, uint32_t n, uint32_t x) { return seg_search(off, 0, n, x); }
-
-// seg_find by a whole wave (all 64 lanes active, same x): each round tests 64
-// evenly spaced candidates at once, so 1,024 segments take 2 dependent loads
-// instead of 10.  off[c] <= x holds for a prefix of c (off is nondecreasing),
+// The segment of x, found by a whole wave (all 64 lanes active, same x): each
+// round tests 64 evenly spaced candidates at once, so 1,024 segments take 2
+// dependent loads instead of 10.  off[c] <= x holds for a prefix of c (off is nondecreasing),
 // so the ballot is a run of low lanes; the last of them brackets the answer.
 TC_DEV uint32_t seg_find_wave(const uint32_t* off, uint32_t n, uint32_t x) {
     uint32_t lo = 0, hi = n;  // answer in [lo, hi), off[lo] <= x
@@ -265,9 +263,9 @@ TC_DEV uint32_t seg_find_wave(const uint32_t* off, uint32_t n, uint32_t x) {
     return lo;
 }
 
-// As seg_find, starting from c0, the segment of an index <= x (a tile's first
-// slot): a few steps forward cover the usual case of a tile inside one or two
-// clouds, a binary search the rest.
+// The segment of x, starting from c0, the segment of an index <= x (a tile's
+// first slot): a few steps forward cover the usual case of a tile inside one or
+// two clouds, a binary search the rest.
 TC_DEV uint32_t seg_find_near(const uint32_t* off, uint32_t n, uint32_t x, uint32_t c0) {
     uint32_t c = c0;
     for (int k = 0; k < 4; ++k) {
