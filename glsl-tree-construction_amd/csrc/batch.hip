@@ -514,7 +514,9 @@ int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s
     TC_TRY(tc_fill_u32(ctx, b->seg_hist, 0u, (size_t)C * DIG * 256, s));
     TC_TRY(tc_fill_u32(ctx, b->ctrs, 0u, 16, s));
     TC_TRY(tc_fill_u32(ctx, b->lb, 0u, (size_t)b->cap_tiles * 256, s));
-    uint32_t grid = (uint32_t)ctx->num_cus * 4;
+    // 16 workgroups per CU, each a contiguous run of tiles (4 per CU: 0.667 ms
+    // per replay of the 1,024-cloud batch; 16: 0.646 ms).
+    uint32_t grid = (uint32_t)ctx->num_cus * 16;
     if (grid > b->cap_tiles) grid = b->cap_tiles;
     hipLaunchKernelGGL(k_seg_morton_hist<K>, dim3(grid), dim3(SEG_BLOCK), 0, s, p,
                        reinterpret_cast<const float4*>(pts), codes, b->seg_hist, b->tile);
