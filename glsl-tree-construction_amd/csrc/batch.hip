@@ -70,7 +70,7 @@ namespace {
 constexpr int SEG_BLOCK = 256;
 // Workgroups per CU of the segmented tree kernels (SEG_BLOCK = one wave per
 // SIMD, so also waves per SIMD): their launch bounds ask the compiler for this
-// occupancy and their grids hold exactly this many per CU.  The octree kernel
+// occupancy and their grids hold four times this many per CU.  The octree kernel
 // fits 5 without spills (96 VGPRs); at 8 (27 registers spilled) it is slower.
 constexpr int SEG_BRT_PER_CU = 8;
 constexpr int SEG_OCT_PER_CU = 5;
@@ -538,7 +538,7 @@ int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s
     TC_LAUNCH_CHECK();
     // The u64 radix-tree kernel's two LDS windows (24 KiB) fit 6 per CU.
     const int brt_per_cu = sizeof(K) == 4 ? SEG_BRT_PER_CU : 6;
-    const uint32_t tgrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK, brt_per_cu);
+    const uint32_t tgrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK, 4 * brt_per_cu);
     hipLaunchKernelGGL(k_seg_brt<K>, dim3(tgrid), dim3(SEG_BLOCK), 0, s, p, b->ubase, uniq, b->prefix_n,
                        b->hll, b->hlr, b->left, b->parent, b->ec);
     TC_LAUNCH_CHECK();
@@ -548,7 +548,7 @@ int batch_pipeline(tc_batch* b, const float* pts, int link_leaves, hipStream_t s
                        b->totals + 1);
     TC_LAUNCH_CHECK();
     const uint32_t ncap = (uint32_t)(b->node_cap < 0xFFFFFFFFull ? b->node_cap : 0xFFFFFFFFull);
-    const uint32_t ogrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK, SEG_OCT_PER_CU);
+    const uint32_t ogrid = (uint32_t)tc_persistent_grid(ctx, b->cap_points, SEG_BLOCK, 4 * SEG_OCT_PER_CU);
     hipLaunchKernelGGL(k_seg_octree<K>, dim3(ogrid), dim3(SEG_BLOCK), 0, s, p.desc, b->ubase, b->node_base,
                        b->nodes, ncap, b->offsets, b->ec, uniq, b->prefix_n, b->left, b->hll, b->hlr, link_leaves, p);
     TC_LAUNCH_CHECK();

@@ -191,9 +191,11 @@ int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void
                    uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, const uint32_t* seg_off,
                    uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s);
 // Grid of a kernel that loops over its work (grid-stride): one workgroup per
-// `block` items, at most per_cu per CU -- the number that is resident at once
-// (a workgroup that does not fit would start only after others finished all
-// their work, a tail as long as the whole kernel).
+// `block` items, at most per_cu per CU.  per_cu is the kernel's resident count
+// or a multiple of it: a grid slightly larger than what fits leaves the extra
+// workgroups to run after the others have finished all their work (a tail as
+// long as the kernel), while four times the resident count makes each
+// workgroup's share small enough for the hardware to balance the load.
 int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu = 8);
 
 // ---------------------------------------------------------------- device side

@@ -97,9 +97,9 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_edge_count(const uint8_t* __rest
 }
 
 template <typename K>
-// Eight workgroups per CU (the persistent grid's count; 64 VGPRs, 14 spilled):
-// at its natural 79 VGPRs only 6 fit and the remaining quarter of the grid
-// would run after the rest (measured: 1.78 -> 1.52 ms on 65 M radix nodes).
+// Eight workgroups per CU resident (64 VGPRs, 14 spilled): at its natural 79
+// VGPRs only 6 fit, and with a grid of 8 per CU the remaining quarter ran after
+// the rest (measured: 1.78 -> 1.52 ms on 65 M radix nodes).
 __global__ __launch_bounds__(TREE_BLOCK, 8) void k_octree_build(
     tc_oct_node* __restrict__ nodes, uint32_t node_cap, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ ec, const K* __restrict__ codes, const uint8_t* __restrict__ prefix_n,
@@ -154,7 +154,10 @@ int brt_impl(tc_ctx* ctx, const K* codes, uint32_t n_keys, const uint32_t* d_n_k
     if (n_keys > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
     if (n_keys < 2) return TC_OK;
     ProfScope prof(ctx, s, K_BRT);
-    const int grid = tc_persistent_grid(ctx, ceil_div_u32(n_keys - 1, BRT_TILE), 1);
+    // Four times the resident workgroups (8 per CU): on 4,761 tiles a grid of
+    // 8 per CU runs whole tiles in 3 rounds where 2.3 suffice (0.080 -> 0.061
+    // ms at the 2^26 Gaussian-mixture build).
+    const int grid = tc_persistent_grid(ctx, ceil_div_u32(n_keys - 1, BRT_TILE), 1, 32);
     hipLaunchKernelGGL(k_build_radix_tree<K>, dim3(grid), dim3(TREE_BLOCK), 0, s, codes,
                        n_keys, d_n_keys, prefix_n, hll, hlr, left, parent, ec);
     TC_LAUNCH_CHECK();
@@ -172,7 +175,7 @@ int octree_impl(tc_ctx* ctx, tc_oct_node* nodes, uint32_t cap, const uint32_t* o
     if (!nodes || !offsets || !ec || !codes || !prefix_n || !parent) return TC_ERR_INVALID;
     if (!left || !hll || !hlr) return TC_ERR_INVALID;
     if (reinterpret_cast<uintptr_t>(nodes) & 15u) return TC_ERR_INVALID;
-    const int grid = tc_persistent_grid(ctx, ceil_div_u32(n_brt, OCT_TILE), 1);
+    const int grid = tc_persistent_grid(ctx, ceil_div_u32(n_brt, OCT_TILE), 1, 32);  // 4 x resident
     ProfScope prof(ctx, s, K_OCT_EMIT);
     hipLaunchKernelGGL(k_octree_build<K>, dim3(grid), dim3(TREE_BLOCK), 0, s, nodes, cap, offsets, ec, codes,
                        prefix_n, left, hll, hlr, n_brt, d_n_keys, link_leaves, min_coord, range);
