@@ -365,9 +365,8 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     t->min_coord = min_coord;
     t->range = range;
     t->link_leaves = link_leaves ? 1 : 0;
-    // Counts start at zero so that an empty cloud reads back as empty.
-    TC_TRY(tc_fill_u32(ctx, t->d_counts, 0u, 4, s));
-    if (n == 0) return TC_OK;
+    // An empty cloud reads back as empty; otherwise unique writes the count.
+    if (n == 0) return tc_fill_u32(ctx, t->d_counts, 0u, 4, s);
     const uint32_t node_cap = (uint32_t)std::min<uint64_t>(t->node_capacity, 0xFFFFFFFFull);
     // morton -> sort -> unique (count stays on the device) -> radix tree with
     // fused edge counts -> exclusive scan -> octree.  Stages after unique read
