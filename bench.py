@@ -349,8 +349,8 @@ def bench_served(ctx, rates=(1000, 2000, 5000, 7500, 10000, 12000, 14000, 16000,
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--log2n", type=int, default=26)
     ap.add_argument("--seed", type=int, default=0xC0FFEE02)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -232,6 +232,29 @@ def test_fused_pipeline_matches_oracle(gpu_ctx, oracle, tclib):
         tree.free()
 
 
+def test_tree_counts_after_empty_and_refill(gpu_ctx, oracle, tclib):
+    """The device counters are zeroed only for an empty cloud (unique writes
+    the count otherwise): a build of 0 points after a full one reads back as
+    empty, and the next non-empty build reads back its own counts."""
+    import tc
+
+    tree = tc.Tree(gpu_ctx, 4096, 30)
+    try:
+        for n, seed in ((4096, 31), (0, 0), (7, 32), (0, 0), (1, 33), (2500, 34)):
+            pts = tclib.gen_uniform(max(n, 1), seed)
+            tree.build(dev(pts), n, 0.0, 1024.0)
+            got = tree.read_counts()
+            if n == 0:
+                assert (got["n_unique"], got["n_brt"], got["n_nodes"]) == (0, 0, 0)
+                continue
+            want = oracle_all(oracle, pts[:n], 0.0, 1024.0, 30)
+            assert got["n_unique"] == len(want["unique"])
+            full = tree.download()
+            assert_same(full, want, ("sorted", "unique", "offsets"))
+    finally:
+        tree.free()
+
+
 def test_fused_pipeline_63bit(gpu_ctx, oracle, tclib):
     import tc
 
