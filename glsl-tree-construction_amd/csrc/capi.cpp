@@ -373,6 +373,8 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
     // the unique count from d_counts; the scan always writes offsets[n_brt].
     // With a point index the sort carries each point's index as its payload and
     // unique also records where each code's run starts (the leaf -> points map).
+    // The scans skip their one-lane epoch kernel: the sort's histogram-finish
+    // kernel and the radix-tree kernel (launched when n >= 2) advance the tag.
     const bool indexed = t->point_index != nullptr;
     if (t->code_bits == 30) {
         uint32_t* sorted = static_cast<uint32_t*>(t->sorted);
@@ -380,14 +382,17 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
         if (indexed) {
             TC_TRY(tc_morton_sort_pairs_u32(ctx, pts, sorted, t->point_index, static_cast<uint32_t*>(t->alt),
                                             t->index_alt, n, min_coord, range, stream));
-            TC_TRY(tc_unique_runs_u32(ctx, sorted, uniq, t->leaf_first, n, t->d_counts, stream));
+            TC_TRY(tc_launch_scan(ctx, MODE_UNIQUE32, sorted, nullptr, uniq, t->leaf_first, t->d_counts, n,
+                                  nullptr, nullptr, 0, nullptr, nullptr, s, true));
         } else {
             TC_TRY(tc_morton_sort_u32(ctx, pts, sorted, static_cast<uint32_t*>(t->alt), n, min_coord, range, stream));
-            TC_TRY(tc_unique_u32(ctx, sorted, uniq, n, t->d_counts, stream));
+            TC_TRY(tc_launch_scan(ctx, MODE_UNIQUE32, sorted, nullptr, uniq, nullptr, t->d_counts, n,
+                                  nullptr, nullptr, 0, nullptr, nullptr, s, true));
         }
         TC_TRY(tc_build_radix_tree_ec_u32(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
                                           t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
-        TC_TRY(tc_exclusive_scan_u32(ctx, t->edge_count, t->offsets, n - 1, t->d_counts, stream));
+        TC_TRY(tc_launch_scan(ctx, MODE_SCAN, t->edge_count, nullptr, t->offsets, nullptr, nullptr, n - 1,
+                              t->d_counts, nullptr, 0, nullptr, nullptr, s, n >= 2));
         if (n >= 2)
             TC_TRY(tc_build_octree_u32(ctx, t->nodes, node_cap, t->offsets, t->edge_count, uniq, t->prefix_n,
                                        t->parent, t->left_child, t->has_leaf_left, t->has_leaf_right, n - 1,
@@ -398,14 +403,17 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
         if (indexed) {
             TC_TRY(tc_morton_sort_pairs_u64(ctx, pts, sorted, t->point_index, static_cast<uint64_t*>(t->alt),
                                             t->index_alt, n, min_coord, range, stream));
-            TC_TRY(tc_unique_runs_u64(ctx, sorted, uniq, t->leaf_first, n, t->d_counts, stream));
+            TC_TRY(tc_launch_scan(ctx, MODE_UNIQUE64, sorted, nullptr, uniq, t->leaf_first, t->d_counts, n,
+                                  nullptr, nullptr, 0, nullptr, nullptr, s, true));
         } else {
             TC_TRY(tc_morton_sort_u64(ctx, pts, sorted, static_cast<uint64_t*>(t->alt), n, min_coord, range, stream));
-            TC_TRY(tc_unique_u64(ctx, sorted, uniq, n, t->d_counts, stream));
+            TC_TRY(tc_launch_scan(ctx, MODE_UNIQUE64, sorted, nullptr, uniq, nullptr, t->d_counts, n,
+                                  nullptr, nullptr, 0, nullptr, nullptr, s, true));
         }
         TC_TRY(tc_build_radix_tree_ec_u64(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
                                           t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
-        TC_TRY(tc_exclusive_scan_u32(ctx, t->edge_count, t->offsets, n - 1, t->d_counts, stream));
+        TC_TRY(tc_launch_scan(ctx, MODE_SCAN, t->edge_count, nullptr, t->offsets, nullptr, nullptr, n - 1,
+                              t->d_counts, nullptr, 0, nullptr, nullptr, s, n >= 2));
         if (n >= 2)
             TC_TRY(tc_build_octree_u64(ctx, t->nodes, node_cap, t->offsets, t->edge_count, uniq, t->prefix_n,
                                        t->parent, t->left_child, t->has_leaf_left, t->has_leaf_right, n - 1,

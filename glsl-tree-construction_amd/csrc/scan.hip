@@ -416,6 +416,8 @@ void launch(uint32_t grid, hipStream_t s, const ScanArgs& a) {
 
 }  // namespace
 
+uint32_t* tc_scan_epoch(tc_ctx* ctx) { return &reinterpret_cast<ScanCtl*>(ctx->small + SMALL_SCAN_CTR)->seq; }
+
 size_t tc_scan_scratch_bytes(uint32_t n) { return ((size_t)ceil_div_u32(n ? n : 1, SCAN_TILE) + 1) * 8; }
 
 int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu) {
@@ -426,7 +428,8 @@ int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu) 
 
 int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0, void* out1,
                    uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, const uint32_t* seg_off,
-                   uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s) {
+                   uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s,
+                   bool epoch_advanced) {
     const uint32_t tiles = ceil_div_u32(n ? n : 1, SCAN_TILE);
     if (!ctx->scan_status) return TC_ERR_NOMEM;
     // Status words for 2^18 tiles (more than 2^30 / SCAN_TILE) are preallocated with the context.
@@ -435,8 +438,10 @@ int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void
     ScanArgs a{in0, in1, out0, out1, count_out, n, d_n_keys, seg_off, n_seg_cap, d_n_seg, seg_out,
                ctx->scan_status, ctl, ctx->spin_limit};
     ProfScope prof(ctx, s, is_seg(mode) ? K_SEG : mode == MODE_SCAN ? K_SCAN : mode == MODE_ECSCAN ? K_ECSCAN : K_UNIQUE);
-    hipLaunchKernelGGL(k_scan_prep, dim3(1), dim3(1), 0, s, ctl);
-    TC_LAUNCH_CHECK();
+    if (!epoch_advanced) {
+        hipLaunchKernelGGL(k_scan_prep, dim3(1), dim3(1), 0, s, ctl);
+        TC_LAUNCH_CHECK();
+    }
     switch (mode) {
         case MODE_SCAN: launch<MODE_SCAN>(tiles, s, a); break;
         case MODE_UNIQUE32: launch<MODE_UNIQUE32>(tiles, s, a); break;

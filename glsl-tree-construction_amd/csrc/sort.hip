@@ -69,10 +69,14 @@ TC_DEV void hist_merge(const uint32_t* s_h, uint32_t* __restrict__ acc) {
 // One workgroup per digit: the digit's counts summed over the accumulator
 // copies, written as an exclusive prefix (digit-major, the layout of
 // shader/histogram.comp:103-195); the copies are cleared for the next use.
+// It also advances the scan epoch tag (as a scan's own one-lane epoch kernel
+// would), so a pipeline whose next scan follows this sort can skip that launch.
 __global__ __launch_bounds__(HIST_BLOCK) void k_hist_finish(uint32_t* __restrict__ acc,
-                                                            uint32_t* __restrict__ hist_out) {
+                                                            uint32_t* __restrict__ hist_out,
+                                                            uint32_t* __restrict__ epoch_seq) {
     __shared__ uint32_t s_tmp[HIST_BLOCK / 64 + 2];
     const uint32_t d = blockIdx.x, tid = threadIdx.x;
+    if (d == 0 && tid == 0) *epoch_seq = (*epoch_seq + 1u) & 0x3FFFFFFFu;
     uint32_t c = 0;
 #pragma unroll 8
     for (int r = 0; r < HIST_COPIES; ++r) {
@@ -288,7 +292,8 @@ int histogram_impl(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist_out, u
     hipLaunchKernelGGL((k_global_histogram<K>), dim3(hist_grid<K>(ctx, n)), dim3(HIST_BLOCK), 0, s, keys, n,
                        ctx->hist_acc, zero_ptr, zero_words);
     TC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_hist_finish, dim3(sizeof(K)), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, hist_out);
+    hipLaunchKernelGGL(k_hist_finish, dim3(sizeof(K)), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, hist_out,
+                       tc_scan_epoch(ctx));
     TC_LAUNCH_CHECK();
     return TC_OK;
 }
@@ -318,7 +323,8 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
                            reinterpret_cast<const float4*>(pts), keys, vals, n, min_coord, range, ctx->hist_acc,
                            ss.ctr, 64 + region_words);
         TC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_hist_finish, dim3(DIG), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, ss.hist);
+        hipLaunchKernelGGL(k_hist_finish, dim3(DIG), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, ss.hist,
+                           tc_scan_epoch(ctx));
         TC_LAUNCH_CHECK();
     } else {
         TC_TRY(histogram_impl<K>(ctx, keys, n, ss.hist, ss.ctr, 64 + region_words, s));

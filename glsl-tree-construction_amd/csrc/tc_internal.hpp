@@ -189,7 +189,12 @@ enum ScanMode {
 int tc_fill_u32(tc_ctx* ctx, uint32_t* ptr, uint32_t value, size_t count, hipStream_t s);
 int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0, void* out1,
                    uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, const uint32_t* seg_off,
-                   uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s);
+                   uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s,
+                   bool epoch_advanced = false);
+// The scans' epoch tag (device word).  A scan launched with epoch_advanced
+// skips its one-lane epoch kernel: the kernel just before it on the stream has
+// advanced the tag (the sort's k_hist_finish, the radix-tree kernel).
+uint32_t* tc_scan_epoch(tc_ctx* ctx);
 // Grid of a kernel that loops over its work (grid-stride): one workgroup per
 // `block` items, at most per_cu per CU.  per_cu is the kernel's resident count
 // or a multiple of it: a grid slightly larger than what fits leaves the extra

@@ -41,11 +41,14 @@ template <typename K>
 __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
     const K* __restrict__ codes, uint32_t n_keys_cap, const uint32_t* __restrict__ d_n_keys,
     uint8_t* __restrict__ prefix_n, uint8_t* __restrict__ hll, uint8_t* __restrict__ hlr,
-    int32_t* __restrict__ left_child, int32_t* __restrict__ parent, uint32_t* __restrict__ ec) {
+    int32_t* __restrict__ left_child, int32_t* __restrict__ parent, uint32_t* __restrict__ ec,
+    uint32_t* __restrict__ epoch_seq) {
     static_assert(BRT_HALO >= BRT_LIMIT, "bounded searches must stay inside the window");
     __shared__ K s_win[BRT_TILE + 2 * BRT_HALO];
     __shared__ uint32_t s_long[BRT_TILE];
     __shared__ uint32_t s_nlong;
+    // Advance the scan epoch tag for the offsets scan that follows in the build.
+    if (blockIdx.x == 0 && threadIdx.x == 0) *epoch_seq = (*epoch_seq + 1u) & 0x3FFFFFFFu;
     const uint32_t nb = resolve_nodes(n_keys_cap > 0 ? n_keys_cap - 1 : 0, d_n_keys);
     const int64_t nk = (int64_t)nb + 1;
     const uint32_t tiles = ceil_div_u32(nb, BRT_TILE);
@@ -159,7 +162,7 @@ int brt_impl(tc_ctx* ctx, const K* codes, uint32_t n_keys, const uint32_t* d_n_k
     // ms at the 2^26 Gaussian-mixture build).
     const int grid = tc_persistent_grid(ctx, ceil_div_u32(n_keys - 1, BRT_TILE), 1, 32);
     hipLaunchKernelGGL(k_build_radix_tree<K>, dim3(grid), dim3(TREE_BLOCK), 0, s, codes,
-                       n_keys, d_n_keys, prefix_n, hll, hlr, left, parent, ec);
+                       n_keys, d_n_keys, prefix_n, hll, hlr, left, parent, ec, tc_scan_epoch(ctx));
     TC_LAUNCH_CHECK();
     return TC_OK;
 }
