@@ -391,7 +391,7 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
         }
         TC_TRY(tc_build_radix_tree_ec_u32(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
                                           t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
-        TC_TRY(tc_launch_scan(ctx, MODE_SCAN, t->edge_count, nullptr, t->offsets, nullptr, nullptr, n - 1,
+        TC_TRY(tc_launch_scan(ctx, MODE_SCAN, t->edge_count, nullptr, t->offsets, nullptr, t->d_counts + 1, n - 1,
                               t->d_counts, nullptr, 0, nullptr, nullptr, s, n >= 2));
         if (n >= 2)
             TC_TRY(tc_build_octree_u32(ctx, t->nodes, node_cap, t->offsets, t->edge_count, uniq, t->prefix_n,
@@ -412,7 +412,7 @@ extern "C" int tc_build_tree(tc_ctx* ctx, const float* pts, uint32_t n, float mi
         }
         TC_TRY(tc_build_radix_tree_ec_u64(ctx, uniq, n, t->d_counts, t->prefix_n, t->has_leaf_left,
                                           t->has_leaf_right, t->left_child, t->parent, t->edge_count, stream));
-        TC_TRY(tc_launch_scan(ctx, MODE_SCAN, t->edge_count, nullptr, t->offsets, nullptr, nullptr, n - 1,
+        TC_TRY(tc_launch_scan(ctx, MODE_SCAN, t->edge_count, nullptr, t->offsets, nullptr, t->d_counts + 1, n - 1,
                               t->d_counts, nullptr, 0, nullptr, nullptr, s, n >= 2));
         if (n >= 2)
             TC_TRY(tc_build_octree_u64(ctx, t->nodes, node_cap, t->offsets, t->edge_count, uniq, t->prefix_n,

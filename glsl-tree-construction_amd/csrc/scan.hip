@@ -405,7 +405,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         if (is_plain_values(MODE)) static_cast<uint32_t*>(a.out0)[n] = total;
         if (MODE == MODE_ECSCAN) static_cast<uint32_t*>(a.out1)[n] = total;
         if (is_unique(MODE) && a.out1) static_cast<uint32_t*>(a.out1)[total] = n;  // end of the last run
-        if (a.count_out) *a.count_out = total;
+        // MODE_SCAN reports [element count, total] (the build's n_brt and
+        // octree node count); the other modes their total.
+        if (a.count_out) {
+            if (MODE == MODE_SCAN) {
+                a.count_out[0] = n;
+                a.count_out[1] = total;
+            } else {
+                *a.count_out = total;
+            }
+        }
     }
 }
 

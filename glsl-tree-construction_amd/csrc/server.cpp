@@ -127,6 +127,10 @@ bool launch(tc_server* s, Slot& sl, std::vector<SchedReq>& batch, std::vector<ui
     if (st == TC_OK) st = tc_batch_replay(sl.batch, sl.stream);
     if (st == TC_OK) st = tc_batch_read_info_async(sl.batch, sl.h_ub, sl.h_nb, sl.stream);
     if (st == TC_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = TC_ERR_HIP;
+    // A launch that failed part-way may have queued kernels that still write
+    // into the requests' result ranges: wait for the stream before the caller
+    // returns those ranges to the arena.
+    if (st != TC_OK) (void)hipStreamSynchronize(sl.stream);
     lk.lock();
     for (Live& l : sl.reqs) l.t_start = t;
     ++s->batches;
@@ -190,6 +194,7 @@ void scheduler_loop(tc_server* s) {
                 complete(s, sl, true);
                 progressed = true;
             } else if (q != hipErrorNotReady) {
+                (void)hipStreamSynchronize(sl.stream);  // nothing may still write the result ranges
                 complete(s, sl, false);
                 progressed = true;
             } else {

@@ -317,6 +317,9 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
     TC_TRY(sort_scratch(ctx, n, DIG, sh, &ss, &tiles));
     const uint32_t region_words = tiles * 256;
     // The histogram kernel also clears the tile counters and look-back region 0.
+    // Every n > 0 sort launches k_hist_finish (both branches below), which
+    // advances the scan epoch: tc_build_tree's unique scan relies on that to
+    // skip its own epoch kernel.
     if (pts) {
         ProfScope prof(ctx, s, K_MORTON);
         hipLaunchKernelGGL((k_morton_hist<K>), dim3(hist_grid<K>(ctx, n)), dim3(HIST_BLOCK), 0, s,
@@ -581,6 +584,14 @@ extern "C" int tc_set_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, int
     ctx->tune[key_bits == 64 ? 1 : 0][pairs ? 1 : 0][log2_n] = (int8_t)idx;
     ctx->tune_on = true;
     return TC_OK;
+}
+
+extern "C" int tc_sort_shape_list(int* blocks, int* kpts, int max) {
+    for (int i = 0; i < N_SORT_SHAPES && i < max; ++i) {
+        if (blocks) blocks[i] = SORT_SHAPES[i].block;
+        if (kpts) kpts[i] = SORT_SHAPES[i].kpt;
+    }
+    return N_SORT_SHAPES;
 }
 
 extern "C" int tc_clear_sort_tuning(tc_ctx* ctx) {

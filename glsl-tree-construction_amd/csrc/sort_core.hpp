@@ -229,7 +229,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
         while (true) {
             uint32_t v[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = (k >= first + j) ? ld_acquire_agent(&lb_cur[(k - j) * LB_ROW + tid]) : 0u;
+            for (int j = 0; j < 4; ++j) v[j] = (k >= first + j) ? ld_agent(&lb_cur[(k - j) * LB_ROW + tid]) : 0u;
             bool found = false;
             uint32_t used = 0;
 #pragma unroll

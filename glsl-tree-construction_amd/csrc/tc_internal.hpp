@@ -186,6 +186,8 @@ enum ScanMode {
 };
 
 // Entry points shared between translation units.
+// tc_launch_scan: count_out (may be NULL) receives the scan's total, except in
+// MODE_SCAN, where count_out[0] = element count and count_out[1] = total.
 int tc_fill_u32(tc_ctx* ctx, uint32_t* ptr, uint32_t value, size_t count, hipStream_t s);
 int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void* out0, void* out1,
                    uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, const uint32_t* seg_off,
@@ -204,11 +206,7 @@ uint32_t* tc_scan_epoch(tc_ctx* ctx);
 int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu = 8);
 
 // ---------------------------------------------------------------- device side
-#ifdef __HIP_DEVICE_COMPILE__
 #define TC_DEV __device__ __forceinline__
-#else
-#define TC_DEV __device__ __forceinline__
-#endif
 
 TC_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
@@ -321,7 +319,10 @@ TC_DEV void lds_count_digits(uint32_t* h, K key) {
     }
 }
 
-TC_DEV uint32_t ld_acquire_agent(const uint32_t* p) {
+// Relaxed agent-scope loads and stores of look-back and status words: the
+// flag and the count share one word, so a reader that sees the flag also sees
+// its count and no acquire ordering is needed.
+TC_DEV uint32_t ld_agent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 TC_DEV void st_agent(uint32_t* p, uint32_t v) {

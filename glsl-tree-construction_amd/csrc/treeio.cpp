@@ -210,6 +210,7 @@ int validate(const tc_tree_host* t, const Fail& fail) {
             } else if (t->link_leaves && ch != 0) {
                 // A leaf slot: the leaf's code has this cell's prefix and digit s.
                 if (ch < 0 || (uint32_t)ch >= U) return fail("octree node %u slot %d: leaf %d out of range", it.node, s, ch);
+                if (it.level + 1 > B / 3) return fail("octree node %u has a leaf slot below the last level", it.node);
                 const int L = it.level + 1;
                 if (((uint64_t)u[ch] >> (B - 3 * L)) != ((it.prefix << 3) | (uint64_t)s))
                     return fail("leaf %d is linked into a cell that does not contain it", ch);

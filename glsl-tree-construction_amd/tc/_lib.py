@@ -226,6 +226,7 @@ SIGNATURES = {
     "tc_tuned_sort_shape": (I, [P, I, I, U32, C.POINTER(I), C.POINTER(I)]),
     "tc_set_tuned_sort_shape": (I, [P, I, I, I, I, I]),
     "tc_clear_sort_tuning": (I, [P]),
+    "tc_sort_shape_list": (I, [C.POINTER(I), C.POINTER(I), I]),
     "tc_morton30": (I, [P, P, P, U32, F, F, S]),
     "tc_morton63": (I, [P, P, P, U32, F, F, S]),
     "tc_morton30_decode": (I, [P, P, P, U32, F, F, S]),

@@ -16,6 +16,14 @@ from ._lib import check, lib
 VARIANTS = {"u32_keys": (32, 0), "u32_pairs": (32, 1), "u64_keys": (64, 0), "u64_pairs": (64, 1)}
 
 
+def built_shapes() -> list:
+    """The (block, keys per thread) launch shapes the library is built with."""
+    k = lib().tc_sort_shape_list(None, None, 0)
+    b, t = (C.c_int * k)(), (C.c_int * k)()
+    lib().tc_sort_shape_list(b, t, k)
+    return list(zip(b, t))
+
+
 def autotune(ctx, max_n: int = 1 << 24) -> dict:
     check(lib().tc_autotune_sort(ctx.handle, max_n), "tc_autotune_sort")
     return table(ctx, max_n)

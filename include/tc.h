@@ -91,6 +91,10 @@ int tc_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, uint32_t n, int* b
 /* Restore one entry of a saved tuning table (size bucket ceil(log2 n)). */
 int tc_set_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, int log2_n, int block, int keys_per_thread);
 int tc_clear_sort_tuning(tc_ctx* ctx);
+/* The binning-pass launch shapes the library is built with (the tuner's
+ * choices): writes up to max (block, keys per thread) pairs and returns how
+ * many there are. */
+int tc_sort_shape_list(int* blocks, int* keys_per_thread, int max);
 /* Bound on a workgroup's look-back wait (empty polls of a predecessor's
  * status) in the sort and scan kernels; past it the workgroup computes its
  * prefix from the input itself, so progress never depends on workgroup start
@@ -282,7 +286,9 @@ typedef struct {
     uint32_t* edge_count;     /* [capacity] */
     uint32_t* offsets;        /* [capacity+1] */
     tc_oct_node* nodes;       /* [node_capacity] */
-    uint32_t* d_counts;       /* device [4]: n_unique, n_brt, n_nodes, reserved */
+    uint32_t* d_counts;       /* device [4], written by every tc_build_tree on the
+                                 device (no host sync): n_unique, n_brt, n_nodes;
+                                 word 3 is reserved (not written) */
     /* filled by tc_tree_read_counts */
     uint32_t n_points, n_unique, n_brt, n_nodes;
     /* Point -> leaf map (NULL unless allocated with TC_TREE_POINT_INDEX).  No

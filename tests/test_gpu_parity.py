@@ -255,6 +255,79 @@ def test_tree_counts_after_empty_and_refill(gpu_ctx, oracle, tclib):
         tree.free()
 
 
+def read_device_counts(tree):
+    import ctypes as C
+
+    import torch
+
+    from tc.api import _hip
+
+    torch.cuda.synchronize()
+    out = np.zeros(4, np.uint32)
+    assert _hip().hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(tree.t.d_counts), C.c_size_t(16), 2) == 0
+    return out
+
+
+def test_device_counts_words_after_shrinking_builds(gpu_ctx, oracle, tclib):
+    """tc_tree.d_counts holds n_unique, n_brt and the octree node count on the
+    device after every build, also when a small build follows a larger one."""
+    import tc
+
+    tree = tc.Tree(gpu_ctx, 300_000, 30)
+    try:
+        for n, seed, gauss in ((300_000, 41, True), (20_000, 42, False), (1, 43, False), (2, 44, False),
+                               (0, 0, False), (5000, 45, True)):
+            pts = tclib.gen_gaussian_mixture(max(n, 1), seed) if gauss else tclib.gen_uniform(max(n, 1), seed)
+            tree.build(dev(pts), n, 0.0, 1024.0)
+            w = read_device_counts(tree)
+            if n == 0:
+                assert list(w[:3]) == [0, 0, 0]
+                continue
+            want = oracle.build_tree(pts[:n], 0.0, 1024.0, 30)
+            u = len(want["unique"])
+            assert list(w[:3]) == [u, max(u - 1, 0), len(want["nodes"])], (n, w)
+    finally:
+        tree.free()
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_unlinked_octree_matches_oracle(gpu_ctx, oracle, tclib, bits):
+    """link_leaves=0 (the reference's default: k_LinkLeafNodes disabled,
+    shader/octree.comp:229-236) gives the oracle's unlinked nodes, plain and
+    batched."""
+    import torch
+
+    import tc
+
+    rg = 1024.0 if bits == 30 else 2097152.0
+    clouds = [tclib.gen_gaussian_mixture(200_000, 71, 16, 4.0 * rg / 1024.0, rg), tclib.gen_uniform(70_000, 72, rg),
+              tclib.gen_uniform(3, 73, rg)]
+    tree = tc.Tree(gpu_ctx, 200_000, bits)
+    try:
+        wants = []
+        for pts in clouds:
+            tree.build(dev(pts), len(pts), 0.0, rg, link_leaves=False)
+            got = tree.download()
+            w_linked = oracle.build_tree(pts, 0.0, rg, bits, link_leaves=True)["nodes"]
+            want = oracle.build_tree(pts, 0.0, rg, bits, link_leaves=False)
+            wants.append(want)
+            assert_same(got, want, ("sorted", "unique", "prefix_n", "left_child", "parent", "edge_count", "offsets"))
+            assert np.array_equal(got["nodes"].view(np.uint8).reshape(-1), want["nodes"].view(np.uint8).reshape(-1))
+            if len(want["nodes"]) > 1:
+                assert not np.array_equal(want["nodes"].view(np.uint8), w_linked.view(np.uint8))
+    finally:
+        tree.free()
+    b = tc.Batch(gpu_ctx, 4, 300_000, bits)
+    try:
+        b.set(np.array([len(c) for c in clouds], np.uint32), None, np.full(len(clouds), rg, np.float32))
+        b.build(dev(np.concatenate(clouds)), link_leaves=False)
+        torch.cuda.synchronize()
+        for i, (g, w) in enumerate(zip(b.download(), wants)):
+            assert np.array_equal(g["nodes"].view(np.uint8).reshape(-1), w["nodes"].view(np.uint8).reshape(-1)), i
+    finally:
+        b.close()
+
+
 def test_fused_pipeline_63bit(gpu_ctx, oracle, tclib):
     import tc
 

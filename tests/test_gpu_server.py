@@ -1,6 +1,8 @@
-"""Serving on the GPU: every request's result equals a plain single-cloud build
-of the same points (bit for bit, every array), alone and in a mix of
-concurrent requests; admission limits and budget accounting."""
+"""Serving on the GPU: every request's result equals the oracle's single-cloud
+build of the same points (bit for bit, every array; the per-cloud contract of
+src/main.rs:60-170), alone and in a mix of concurrent requests, up to the
+served workload's request sizes (2^12 .. 2^22 points); admission limits and
+budget accounting."""
 import threading
 import time
 
@@ -26,12 +28,20 @@ def plain_build(ctx, pts, code_bits=30, rng=1024.0):
         tree.free()
 
 
+def oracle_tree(oracle, pts, code_bits=30, rng=1024.0):
+    w = oracle.build_tree(pts, 0.0, rng, code_bits)
+    w["n_unique"] = len(w["unique"])
+    w["n_nodes"] = len(w["nodes"])
+    return w
+
+
 def assert_same_tree(got, want, tag=""):
     assert got["n_unique"] == want["n_unique"], tag
     assert got["n_nodes"] == want["n_nodes"], tag
     for k in ARRAYS:
         assert np.array_equal(got[k], want[k]), (tag, k)
-    assert np.array_equal(got["nodes"].view(np.uint8), want["nodes"].view(np.uint8)), (tag, "nodes")
+    assert np.array_equal(got["nodes"].view(np.uint8).reshape(-1), want["nodes"].view(np.uint8).reshape(-1)), \
+        (tag, "nodes")
 
 
 def make_clouds(tclib, k, seed, lo=2 ** 8, hi=2 ** 17):
@@ -57,7 +67,7 @@ def collect(srv, ids, timeout_s=60.0):
     return out
 
 
-def test_served_results_equal_plain_builds(gpu_ctx, tclib):
+def test_served_results_equal_oracle_builds(gpu_ctx, oracle, tclib):
     import tc
 
     clouds = make_clouds(tclib, 48, seed=7)
@@ -73,14 +83,14 @@ def test_served_results_equal_plain_builds(gpu_ctx, tclib):
             res = got[rid]
             assert res["status"] == 0 and res["n_points"] == len(c)
             assert res["t_submit"] <= res["t_start"] <= res["t_done"]
-            assert_same_tree(srv.download(res), plain_build(gpu_ctx, c), f"cloud {i} n={len(c)}")
+            assert_same_tree(srv.download(res), oracle_tree(oracle, c), f"cloud {i} n={len(c)}")
             srv.release(rid)
         assert srv.stats()["reserved_bytes"] == 0
     finally:
         srv.close()
 
 
-def test_result_same_alone_and_in_a_concurrent_mix(gpu_ctx, tclib):
+def test_result_same_alone_and_in_a_concurrent_mix(gpu_ctx, oracle, tclib):
     import tc
 
     probe = tclib.gen_gaussian_mixture(150_000, 99, 16, 4.0)
@@ -96,6 +106,7 @@ def test_result_same_alone_and_in_a_concurrent_mix(gpu_ctx, tclib):
 
         d_others = [dev(c) for c in others]
         ids = []
+        index_of = {}
         lock = threading.Lock()
 
         def client(lo, hi, seed):
@@ -103,7 +114,9 @@ def test_result_same_alone_and_in_a_concurrent_mix(gpu_ctx, tclib):
             for j in range(lo, hi):
                 time.sleep(float(r.uniform(0, 2e-4)))
                 with lock:
-                    ids.append(srv.submit(d_others[j], len(others[j])))
+                    rid = srv.submit(d_others[j], len(others[j]))
+                    ids.append(rid)
+                    index_of[rid] = j
 
         threads = [threading.Thread(target=client, args=(k * 10, k * 10 + 10, k)) for k in range(4)]
         for t in threads:
@@ -114,14 +127,18 @@ def test_result_same_alone_and_in_a_concurrent_mix(gpu_ctx, tclib):
             t.join()
         got = collect(srv, ids + [probe_id])
         assert_same_tree(srv.download(got[probe_id]), alone_tree, "probe in mix")
+        assert_same_tree(alone_tree, oracle_tree(oracle, probe), "probe alone vs oracle")
         assert_same_tree(alone_tree, plain_build(gpu_ctx, probe), "probe alone vs plain")
+        for j, rid in enumerate(ids):
+            c = others[index_of[rid]]
+            assert_same_tree(srv.download(got[rid]), oracle_tree(oracle, c), f"mix request {j} n={len(c)}")
         for rid in got:
             srv.release(rid)
     finally:
         srv.close()
 
 
-def test_63bit_served(gpu_ctx, tclib):
+def test_63bit_served(gpu_ctx, oracle, tclib):
     import tc
 
     clouds = [tclib.gen_uniform(n, 60 + n, 2097152.0) for n in (5000, 70_000, 1)]
@@ -131,7 +148,7 @@ def test_63bit_served(gpu_ctx, tclib):
         ids = [srv.submit(p, len(c), 0.0, 2097152.0) for p, c in zip(d, clouds)]
         got = collect(srv, ids)
         for rid, c in zip(ids, clouds):
-            assert_same_tree(srv.download(got[rid]), plain_build(gpu_ctx, c, 63, 2097152.0), f"n={len(c)}")
+            assert_same_tree(srv.download(got[rid]), oracle_tree(oracle, c, 63, 2097152.0), f"n={len(c)}")
             srv.release(rid)
     finally:
         srv.close()
@@ -200,3 +217,47 @@ def test_trace_runners(gpu_ctx, tclib):
         srv.close()
     lat_p = tc.plain_run_trace(gpu_ctx, t, pts, sizes)
     assert np.all(lat_p > 0) and np.all(lat_p < 5.0)
+
+
+def test_served_workload_sizes_poisson_match_oracle(gpu_ctx, oracle, tclib):
+    """The served workload's request mix: sizes log-uniform in [2^12, 2^22],
+    uniform xyz, Poisson arrivals at 50k req/s from a submitting thread, the
+    server's default configuration (64 GiB budget, 2 streams).  The arrivals
+    outpace the builds, so requests are batched (mean >= 8 per batch); every
+    result is compared with the oracle's build of its points."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import tc
+
+    r = np.random.default_rng(0xC0FFEE05)
+    k = 128
+    sizes = np.round(np.exp(r.uniform(np.log(2 ** 12), np.log(2 ** 22), k))).astype(np.uint32)
+    sizes[0] = 1 << 22  # the largest request of the range
+    clouds = [tclib.gen_uniform(int(s), 0xC0FFEE05 + i) for i, s in enumerate(sizes)]
+    t_arr = np.cumsum(r.exponential(1 / 50_000.0, k))
+    t_arr -= t_arr[0]
+    d = [dev(c) for c in clouds]
+    srv = tc.Server(gpu_ctx)
+    try:
+        ids = [0] * k
+        t0 = time.perf_counter()
+        for i in range(k):
+            while time.perf_counter() - t0 < t_arr[i]:
+                pass
+            ids[i] = srv.submit(d[i], int(sizes[i]))
+        got = collect(srv, ids, timeout_s=120.0)
+        st = srv.stats()
+        print(f"served workload: {k} requests, {int(sizes.sum())} points, {st['batches']} batches, "
+              f"{st['mean_clouds_per_batch']:.1f} clouds per batch, p99 {st['latency_p99_ms']:.2f} ms", flush=True)
+        assert st["completed"] == k
+        assert st["mean_clouds_per_batch"] >= 8.0
+        with ThreadPoolExecutor(16) as ex:
+            want = list(ex.map(lambda c: oracle_tree(oracle, c), clouds))
+        for i, rid in enumerate(ids):
+            res = got[rid]
+            assert res["status"] == 0 and res["n_points"] == sizes[i]
+            assert_same_tree(srv.download(res), want[i], f"request {i} n={sizes[i]}")
+            srv.release(rid)
+        assert srv.stats()["reserved_bytes"] == 0
+    finally:
+        srv.close()

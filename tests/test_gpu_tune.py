@@ -7,18 +7,17 @@ from gpu_util import dev, empty, host
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = {(b, k) for b in (256, 512) for k in (8, 12, 16, 20)}
-
 
 def test_autotune_table_and_sorts(oracle, tclib, tmp_path):
     import tc
 
     ctx = tc.Context(0)
     try:
+        shapes = set(tc.tune.built_shapes())
         t = tc.tune.autotune(ctx, 1 << 20)
         assert set(t) == {"u32_keys", "u32_pairs", "u64_keys", "u64_pairs"}
         for entries in t.values():
-            assert entries and all(tuple(v) in SHAPES for v in entries.values())
+            assert entries and all(tuple(v) in shapes for v in entries.values())
         r = np.random.default_rng(21)
         for n in (1000, 70_000, 1 << 20):
             keys = r.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
