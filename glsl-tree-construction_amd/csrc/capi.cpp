@@ -203,6 +203,8 @@ extern "C" int tc_create(tc_ctx** out, int device) {
         tc_destroy(ctx);
         return TC_ERR_NOMEM;
     }
+    // A failed lookup only disables tc_sort_keys_u32_noalloc (it then returns TC_ERR_HIP).
+    (void)tc_noalloc_init(ctx);
     *out = ctx;
     return TC_OK;
 }

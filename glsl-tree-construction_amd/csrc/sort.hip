@@ -263,11 +263,15 @@ uint32_t hist_grid(tc_ctx* ctx, uint32_t n) {
     return blocks < cap ? blocks : cap;
 }
 
-// Scratch layout for a sort: [16 tile counters][region 0][region 1][hist]
+// Scratch of one sort: [64 tile counters][region 0][region 1][hist], plus the
+// histogram accumulators (zero between uses) and the scan epoch word that the
+// histogram-finish kernel advances.
 struct SortScratch {
     uint32_t* ctr;
     uint32_t* region[2];
     uint32_t* hist;
+    uint32_t* acc;
+    uint32_t* epoch;
 };
 
 int sort_scratch(tc_ctx* ctx, uint32_t n, int digits, SortShape sh, SortScratch* ss, uint32_t* tiles_out) {
@@ -281,56 +285,50 @@ int sort_scratch(tc_ctx* ctx, uint32_t n, int digits, SortShape sh, SortScratch*
     ss->region[0] = base + 64;
     ss->region[1] = base + 64 + region_words;
     ss->hist = base + 64 + 2 * region_words;
+    ss->acc = ctx->hist_acc;
+    ss->epoch = tc_scan_epoch(ctx);
     *tiles_out = tiles;
     return TC_OK;
 }
 
 template <typename K>
-int histogram_impl(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist_out, uint32_t* zero_ptr,
-                   uint32_t zero_words, hipStream_t s) {
+int histogram_impl(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist_out, uint32_t* acc, uint32_t* epoch,
+                   uint32_t* zero_ptr, uint32_t zero_words, hipStream_t s) {
     ProfScope prof(ctx, s, K_HIST);
     hipLaunchKernelGGL((k_global_histogram<K>), dim3(hist_grid<K>(ctx, n)), dim3(HIST_BLOCK), 0, s, keys, n,
-                       ctx->hist_acc, zero_ptr, zero_words);
+                       acc, zero_ptr, zero_words);
     TC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_hist_finish, dim3(sizeof(K)), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, hist_out,
-                       tc_scan_epoch(ctx));
+    hipLaunchKernelGGL(k_hist_finish, dim3(sizeof(K)), dim3(HIST_BLOCK), 0, s, acc, hist_out, epoch);
     TC_LAUNCH_CHECK();
     return TC_OK;
 }
 
-// Sort keys (and values) in place.  With pts != NULL the keys are first
-// computed from the points by the fused Morton + histogram kernel, which also
-// sets vals[i] = i when vals is given (the sorted values are then the
-// permutation from sorted position to point index).
+// The launches of one sort (histogram, then one binning pass per digit) on
+// the scratch `ss` (tiles look-back rows per region).  With pts != NULL the
+// keys are first computed from the points by the fused Morton + histogram
+// kernel, which also sets vals[i] = i when vals is given (the sorted values are
+// then the permutation from sorted position to point index).
 template <typename K>
-int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint32_t n, hipStream_t s,
-              const float* pts = nullptr, float min_coord = 0.f, float range = 1.f) {
-    if (!ctx || (n && (!keys || !alt))) return TC_ERR_INVALID;
-    if (vals && !valt) return TC_ERR_INVALID;
-    if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
-    if (n == 0) return TC_OK;
-    if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
+int sort_run(tc_ctx* ctx, SortShape sh, const SortScratch& ss, uint32_t tiles, K* keys, uint32_t* vals, K* alt,
+             uint32_t* valt, uint32_t n, hipStream_t s, const float* pts, float min_coord, float range) {
     constexpr int DIG = (int)sizeof(K);
-    const SortShape sh = tc_sort_shape(ctx, DIG, vals != nullptr, n);
-    SortScratch ss;
-    uint32_t tiles;
-    TC_TRY(sort_scratch(ctx, n, DIG, sh, &ss, &tiles));
     const uint32_t region_words = tiles * 256;
-    // The histogram kernel also clears the tile counters and look-back region 0.
+    // The histogram kernel also clears the tile counters and look-back region 0,
+    // as one range: the region must follow the 64 counters.
+    if (ss.region[0] != ss.ctr + 64) return TC_ERR_INVALID;
     // Every n > 0 sort launches k_hist_finish (both branches below), which
     // advances the scan epoch: tc_build_tree's unique scan relies on that to
     // skip its own epoch kernel.
     if (pts) {
         ProfScope prof(ctx, s, K_MORTON);
         hipLaunchKernelGGL((k_morton_hist<K>), dim3(hist_grid<K>(ctx, n)), dim3(HIST_BLOCK), 0, s,
-                           reinterpret_cast<const float4*>(pts), keys, vals, n, min_coord, range, ctx->hist_acc,
+                           reinterpret_cast<const float4*>(pts), keys, vals, n, min_coord, range, ss.acc,
                            ss.ctr, 64 + region_words);
         TC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_hist_finish, dim3(DIG), dim3(HIST_BLOCK), 0, s, ctx->hist_acc, ss.hist,
-                           tc_scan_epoch(ctx));
+        hipLaunchKernelGGL(k_hist_finish, dim3(DIG), dim3(HIST_BLOCK), 0, s, ss.acc, ss.hist, ss.epoch);
         TC_LAUNCH_CHECK();
     } else {
-        TC_TRY(histogram_impl<K>(ctx, keys, n, ss.hist, ss.ctr, 64 + region_words, s));
+        TC_TRY(histogram_impl<K>(ctx, keys, n, ss.hist, ss.acc, ss.epoch, ss.ctr, 64 + region_words, s));
     }
     K* src = keys;
     K* dst = alt;
@@ -353,6 +351,22 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
     }
     // An even number of passes leaves the result in keys/vals.
     return TC_OK;
+}
+
+// Sort keys (and values) in place on the context's scratch.
+template <typename K>
+int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint32_t n, hipStream_t s,
+              const float* pts = nullptr, float min_coord = 0.f, float range = 1.f) {
+    if (!ctx || (n && (!keys || !alt))) return TC_ERR_INVALID;
+    if (vals && !valt) return TC_ERR_INVALID;
+    if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (n == 0) return TC_OK;
+    if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
+    const SortShape sh = tc_sort_shape(ctx, (int)sizeof(K), vals != nullptr, n);
+    SortScratch ss;
+    uint32_t tiles;
+    TC_TRY(sort_scratch(ctx, n, (int)sizeof(K), sh, &ss, &tiles));
+    return sort_run<K>(ctx, sh, ss, tiles, keys, vals, alt, valt, n, s, pts, min_coord, range);
 }
 
 template <typename K>
@@ -379,7 +393,7 @@ template <typename K>
 int global_histogram_api(tc_ctx* ctx, const K* keys, uint32_t n, uint32_t* hist, hipStream_t s) {
     if (!ctx || !hist || (n && !keys)) return TC_ERR_INVALID;
     if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
-    return histogram_impl<K>(ctx, keys, n, hist, nullptr, 0, s);
+    return histogram_impl<K>(ctx, keys, n, hist, ctx->hist_acc, tc_scan_epoch(ctx), nullptr, 0, s);
 }
 
 }  // namespace
@@ -444,6 +458,73 @@ extern "C" int tc_sort_pairs_u64(tc_ctx* ctx, uint64_t* keys, uint32_t* vals, ui
                                  uint32_t n, tc_stream stream) {
     if (!vals) return TC_ERR_INVALID;
     return sort_impl<uint64_t>(ctx, keys, vals, alt, valt, n, as_stream(stream));
+}
+
+// ------------------------------------------------- sort that allocates nothing
+// tc_sort_keys_u32_noalloc keeps its scratch in one file-scope device array
+// (each GPU holds its own copy, at its own address, until the process ends):
+// tile counters, the histogram, the scan epoch word, the histogram
+// accumulators and two look-back regions sized for 2^24 keys in tiles of at
+// least 8192 keys.  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device (tc_ctx::noalloc_scratch, set by
+// tc_create with the context's device current); the array's name itself is a
+// host-side stand-in that must never be passed to a kernel.
+namespace {
+
+constexpr uint32_t NA_MAX_N = 1u << 24;
+constexpr uint32_t NA_MIN_TILE = 8192;
+constexpr size_t NA_REGION_WORDS = (size_t)(NA_MAX_N / NA_MIN_TILE) * 256;
+// The sort's layout: the tile counters directly followed by look-back region 0
+// (the histogram kernel clears both as one range).
+constexpr size_t NA_CTR = 0;
+constexpr size_t NA_R0 = 64;
+constexpr size_t NA_R1 = NA_R0 + NA_REGION_WORDS;
+constexpr size_t NA_HIST = NA_R1 + NA_REGION_WORDS;
+constexpr size_t NA_EPOCH = NA_HIST + 8 * 256;
+constexpr size_t NA_ACC = NA_EPOCH + 64;
+constexpr size_t NA_WORDS = NA_ACC + (size_t)HIST_COPIES * 8 * 256;  // 1,116,288 words = 4.26 MiB
+
+static __device__ __align__(256) uint32_t scratch[NA_WORDS];
+
+}  // namespace
+
+int tc_noalloc_init(tc_ctx* ctx) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        ctx->noalloc_scratch = nullptr;
+        return TC_ERR_HIP;
+    }
+    ctx->noalloc_scratch = static_cast<uint32_t*>(ptr);
+    return TC_OK;
+}
+
+extern "C" int tc_sort_keys_u32_noalloc(tc_ctx* ctx, uint32_t* keys, uint32_t* alt, uint32_t n, tc_stream stream) {
+    if (!ctx || (n && (!keys || !alt))) return TC_ERR_INVALID;
+    if (n > NA_MAX_N) return TC_ERR_TOO_LARGE;
+    if (n == 0) return TC_OK;
+    if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
+    uint32_t* base = ctx->noalloc_scratch;
+    if (!base) return TC_ERR_HIP;
+    hipStream_t s = as_stream(stream);
+    // The tuned shape unless its tiles are too small for the static regions;
+    // then the first built shape whose tiles are (every n <= 2^24 fits those).
+    SortShape sh = tc_sort_shape(ctx, 4, false, n);
+    if ((size_t)ceil_div_u32(n, (uint32_t)(sh.block * sh.kpt)) * 256 > NA_REGION_WORDS) {
+        for (const SortShape& c : SORT_SHAPES) {
+            if ((uint32_t)(c.block * c.kpt) >= NA_MIN_TILE) {
+                sh = c;
+                break;
+            }
+        }
+    }
+    const uint32_t tiles = ceil_div_u32(n, (uint32_t)(sh.block * sh.kpt));
+    SortScratch ss{base + NA_CTR, {base + NA_R0, base + NA_R1}, base + NA_HIST, base + NA_ACC, base + NA_EPOCH};
+    // The accumulators must start at zero: every sort leaves them so, but a
+    // call that failed part-way may not have.
+    TC_TRY(tc_fill_u32(ctx, ss.acc, 0u, (size_t)HIST_COPIES * 8 * 256, s));
+    return sort_run<uint32_t>(ctx, sh, ss, tiles, keys, nullptr, alt, nullptr, n, s, nullptr, 0.f, 1.f);
 }
 
 // ------------------------------------------------------------------ autotuner

@@ -101,6 +101,9 @@ struct tc_ctx {
     // Scan tile status words (epoch-tagged, never cleared): one per 8192-element
     // tile up to 2^30 elements.
     uint64_t* scan_status = nullptr;
+    // This device's address of the static scratch array of the sort that
+    // allocates nothing (tc_sort_keys_u32_noalloc); NULL if it was not found.
+    uint32_t* noalloc_scratch = nullptr;
     // Host-side staging for the host-array wrappers.
     std::mutex mu;
 };
@@ -129,6 +132,9 @@ constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {{256, 8},  {512, 8},  {512, 12
 // when the context has been tuned, else the context's fixed shape.
 SortShape tc_sort_shape(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n);
 size_t tc_scan_scratch_bytes(uint32_t n);
+// Looks up this device's address of the no-alloc sort's static scratch
+// (current device = ctx->device); TC_ERR_HIP if the lookup fails.
+int tc_noalloc_init(tc_ctx* ctx);
 
 static inline hipStream_t as_stream(tc_stream s) { return reinterpret_cast<hipStream_t>(s); }
 

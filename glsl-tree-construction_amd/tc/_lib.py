@@ -227,6 +227,7 @@ SIGNATURES = {
     "tc_set_tuned_sort_shape": (I, [P, I, I, I, I, I]),
     "tc_clear_sort_tuning": (I, [P]),
     "tc_sort_shape_list": (I, [C.POINTER(I), C.POINTER(I), I]),
+    "tc_sort_keys_u32_noalloc": (I, [P, P, P, U32, S]),
     "tc_morton30": (I, [P, P, P, U32, F, F, S]),
     "tc_morton63": (I, [P, P, P, U32, F, F, S]),
     "tc_morton30_decode": (I, [P, P, P, U32, F, F, S]),

@@ -185,6 +185,15 @@ int tc_morton_sort_pairs_u64(tc_ctx* ctx, const float* pts_xyzw, uint64_t* keys,
                              float range, tc_stream stream);
 int tc_sort_pairs_u64(tc_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                       uint32_t* vals_alt, uint32_t n, tc_stream stream);
+/* tc_sort_keys_u32 that allocates nothing.  No reference counterpart: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Its scratch is a static
+ * 4.26 MiB device array per GPU, held until the process ends, found at
+ * tc_create; it calls no hipMalloc/hipFree.  n <= 2^24 (TC_ERR_TOO_LARGE
+ * above).  Every call on one GPU shares that array, whatever the context:
+ * calls that overlap on one GPU (other streams, other contexts) race and must
+ * be serialised by the caller.  Same result as tc_sort_keys_u32. */
+int tc_sort_keys_u32_noalloc(tc_ctx* ctx, uint32_t* keys, uint32_t* keys_alt, uint32_t n, tc_stream stream);
 
 /* ---- Unique: k_CountUnique, shader/unique.comp:31-45; host wrapper
  *      unique(), src/unique.rs:37.  out[0..U) = first key of each run of equal
