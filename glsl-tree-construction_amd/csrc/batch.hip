@@ -233,7 +233,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_onesweep(Plan p, const K* __restr
     const uint32_t cnt = end - base < Sh::TILE ? end - base : Sh::TILE;
     onesweep_tile<K, false, BLOCK, KPT, RB>(sm, sk, dk, nullptr, nullptr, base, cnt, shift, tile, first, seg_base,
                                             seg_hist + ((size_t)c * DIG + pass) * 256, seg_base, lb_cur, lb_next,
-                                            spin_limit);
+                                            LB_ROW, spin_limit);
 }
 
 // Fix-ups after a segmented scan: empty segments get the base of the next
@@ -591,7 +591,7 @@ extern "C" int tc_batch_create(tc_ctx* ctx, uint32_t cap_clouds, uint32_t cap_po
     // total key count (measured: 1,024 clouds of 140.9 M points in all take
     // 7.41 ms per replay with 512 x 20 or larger tiles, 7.86 ms with the
     // 512 x 8 tiles tuned for their average size).
-    const SortShape sh = tc_sort_shape(ctx, code_bits == 30 ? 4 : 8, false, cap_points);
+    const SortShape sh = tc_sort_shape_8bit(ctx, code_bits == 30 ? 4 : 8, false, cap_points);
     b->block = sh.block;
     b->kpt = sh.kpt;
     b->tile = (uint32_t)(b->block * b->kpt);

@@ -192,7 +192,7 @@ extern "C" int tc_create(tc_ctx** out, int device) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cus = prop.multiProcessorCount;
     const size_t status_bytes = ((size_t)(1u << 18) + 1) * sizeof(uint64_t);
-    const size_t acc_bytes = (size_t)HIST_COPIES * 8 * 256 * 4;
+    const size_t acc_bytes = tc_hist_acc_bytes();
     if (hipMalloc(&ctx->small, SMALL_WORDS * 4) != hipSuccess ||
         hipMalloc(&ctx->scan_status, status_bytes) != hipSuccess ||
         hipMalloc(&ctx->hist_acc, acc_bytes) != hipSuccess ||
@@ -227,27 +227,25 @@ extern "C" int tc_reserve(tc_ctx* ctx, uint32_t max_n) {
     if (!ctx) return TC_ERR_INVALID;
     if (max_n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
     TC_TRY(set_device(ctx));
-    // Smallest tile of any supported launch shape (256 x 8) needs the most words.
-    return tc_ensure_scratch(ctx, tc_sort_scratch_bytes(max_n, 8, 256 * 8));
+    return tc_ensure_scratch(ctx, tc_sort_scratch_max_bytes(max_n));
 }
 
 extern "C" int tc_set_sort_params(tc_ctx* ctx, int block, int kpt) {
-    if (!ctx) return TC_ERR_INVALID;
-    if (block == 0) block = ctx->sort_block;
-    if (kpt == 0) kpt = ctx->sort_kpt;
-    bool built = false;  // one of the shapes the binning pass is compiled for
-    for (const SortShape& sh : SORT_SHAPES) built = built || (sh.block == block && sh.kpt == kpt);
-    if (!built) return TC_ERR_INVALID;
-    ctx->sort_block = block;
-    ctx->sort_kpt = kpt;
-    ctx->tune_on = false;  // an explicit shape replaces tuned ones
-    return TC_OK;
+    return tc_set_sort_plan(ctx, block, kpt, 8);
 }
 
 extern "C" int tc_get_sort_params(tc_ctx* ctx, int* block, int* kpt) {
     if (!ctx || !block || !kpt) return TC_ERR_INVALID;
     *block = ctx->sort_block;
     *kpt = ctx->sort_kpt;
+    return TC_OK;
+}
+
+extern "C" int tc_get_sort_plan(tc_ctx* ctx, int* block, int* kpt, int* digit_bits) {
+    if (!ctx || !block || !kpt || !digit_bits) return TC_ERR_INVALID;
+    *block = ctx->sort_block;
+    *kpt = ctx->sort_kpt;
+    *digit_bits = ctx->sort_bits;
     return TC_OK;
 }
 

@@ -4,21 +4,28 @@
 // One tile = BLOCK*KPT keys.  Wave w owns keys [w*64*KPT, (w+1)*64*KPT) of the
 // tile, lane-striped (key i*64 + lane), so a wave's processing order (i, lane)
 // is the keys' index order and the rank below is stable.
+//
+// Digit width RB (1..11 bits) is a template parameter: a sort of KB-bit keys
+// runs ceil(KB / RB_max) passes whose widths differ by at most one bit (for
+// 30-bit Morton codes: three 10-bit passes, or 8, 8, 7, 7).  Wider digits mean
+// fewer passes over HBM but more ballots per key and longer look-back rows;
+// the autotuner picks the width with the launch shape.
 #pragma once
 
 #include "tc_internal.hpp"
 
-// Digit width: 8 bits per pass, except that the last pass over Morton codes,
-// whose top digit has fewer live bits (30-bit codes: bits 24..29; 63-bit
-// codes: bits 56..62), ranks only those bits.  Fewer bits mean fewer ballots
-// per key and fewer digits to publish and look back over (measured on MI355X,
-// 2^26 keys: 0.189 ms for an 8-bit pass, 0.175 ms for 7 bits, 0.164 ms for 6).
-// The digit histogram and the look-back rows keep 256 words per digit
-// position / tile whatever the width; a narrow pass uses the first 2^RB.
+// Look-back row stride (u32 words per tile) of an RB-bit pass: one word per
+// digit, at least 256 (the segmented batch sort keeps 256-word rows for its
+// 8-bit and narrower passes).
 constexpr uint32_t LB_ROW = 256;
+template <int RB>
+constexpr uint32_t lb_row() {
+    return (1u << RB) < LB_ROW ? LB_ROW : (1u << RB);
+}
 
 // Live bits of the top digit (shift 8*(bytes-1)) of a Morton code: 30 - 24
-// for 30-bit codes, 63 - 56 for 63-bit codes.
+// for 30-bit codes, 63 - 56 for 63-bit codes (the segmented batch sort's
+// narrow last pass).
 template <typename K>
 constexpr int morton_top_digit_bits() {
     return sizeof(K) == 4 ? 6 : 7;
@@ -65,15 +72,21 @@ TC_DEV uint64_t match_digit(uint32_t word, uint32_t off, uint64_t live) {
 
 template <typename K, int RB = 8>
 TC_DEV uint32_t digit_of(K k, uint32_t shift) {
-    return (uint32_t)((k >> shift) & ((1u << RB) - 1u));
+    return (uint32_t)((k >> shift) & (K)((1u << RB) - 1u));
 }
 
-// The 32-bit half of a key that holds the digit at `shift` (a multiple of 8,
-// so with RB <= 8 the digit never straddles the halves).
+// A 32-bit word holding the RB-bit digit at `shift` at bit offset *off
+// (uniform): u32 keys are their own word; u64 keys are shifted once, since a
+// digit may straddle the two halves.
 template <typename K>
-TC_DEV uint32_t digit_word(K k, uint32_t shift) {
-    if constexpr (sizeof(K) == 4) return (uint32_t)k;
-    else return shift >= 32u ? (uint32_t)(k >> 32) : (uint32_t)k;
+TC_DEV uint32_t digit_word(K k, uint32_t shift, uint32_t* off) {
+    if constexpr (sizeof(K) == 4) {
+        *off = shift;
+        return (uint32_t)k;
+    } else {
+        *off = 0u;
+        return (uint32_t)(k >> shift);
+    }
 }
 
 template <typename K, int BLOCK, int KPT, int RB = 8>
@@ -81,18 +94,19 @@ struct OnesweepShared {
     static constexpr int WAVES = BLOCK / 64;
     static constexpr uint32_t TILE = BLOCK * KPT;
     static constexpr int D = 1 << RB;
-    // Per-wave digit counters, then tile-local positions (< TILE <= 65535,
-    // so 16 bits); the fall-back digit counts reuse the space afterwards.
-    union {
-        uint16_t whist[WAVES * D];
-        uint32_t fb[D];
-    };
-    uint32_t gbase[D];
+    // Digits owned per thread in the per-digit steps (thread t: digits
+    // [t*DPT, t*DPT + DPT)); two are read and published as one 64-bit word.
+    static constexpr int DPT = D > BLOCK ? D / BLOCK : 1;
+    static constexpr size_t KEY_BYTES = (size_t)TILE * sizeof(K);
+    static constexpr size_t WH_BYTES = (size_t)WAVES * D * sizeof(uint16_t);
+    // Per-wave digit counters during the ranking (then tile-local positions,
+    // < TILE <= 65535, so 16 bits), then the keys staged in digit order, then
+    // the values: one region, since their lifetimes do not overlap.
+    alignas(16) unsigned char stage[KEY_BYTES > WH_BYTES ? KEY_BYTES : WH_BYTES];
+    uint32_t gbase[D];  // digit bases of the scatter; the fall-back's digit counts before that
     uint32_t tmp[WAVES + 2];
-    uint32_t tile;
     uint32_t aux;
     uint32_t fallback;
-    alignas(16) unsigned char stage[TILE * sizeof(K)];
 };
 
 // One digit-binning tile step (shader/new_radix_sort.comp:116-314), digit
@@ -100,11 +114,12 @@ struct OnesweepShared {
 //   keys sk[base, base+cnt) (cnt <= TILE) go to
 //   dk[prefix[d] + add + (#digit-d keys of earlier tiles of this segment) + rank].
 // The caller guarantees that the key bits above shift + RB are equal within
-// the segment (zero for Morton codes), so RB < 8 sorts like an 8-bit digit.
-// `tile` indexes the look-back rows (LB_ROW u32 words per tile) of lb_cur; the
-// look-back walks down to `first`, the first tile of the segment, which
-// publishes its counts as inclusive.  lb_next (may be NULL) is the next pass's
-// look-back row of this tile, cleared here.
+// the segment (zero for Morton codes, or the bits of later passes), which the
+// LSD order makes irrelevant here.
+// `tile` indexes the look-back rows (lb_row<RB>() u32 words per tile) of
+// lb_cur; the look-back walks down to `first`, the first tile of the segment,
+// which publishes its counts as inclusive.  lb_next (may be NULL) is the next
+// pass's look-back row of this tile, cleared here (next_row words).
 // seg_base is the index of the segment's first key (the fall-back counts the
 // keys in [seg_base, base)); spin_limit bounds the look-back wait.
 template <typename K, bool PAIRS, int BLOCK, int KPT, int RB, bool FULL, bool STREAM>
@@ -112,11 +127,17 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
                                const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
                                uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first, uint32_t seg_base,
                                const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
-                               uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
-    constexpr int WAVES = BLOCK / 64;
+                               uint32_t* __restrict__ lb_next, uint32_t next_row, uint32_t spin_limit) {
+    using Sh = OnesweepShared<K, BLOCK, KPT, RB>;
+    constexpr int WAVES = Sh::WAVES;
     constexpr uint32_t WAVE_TILE = 64 * KPT;
     constexpr uint32_t D = 1u << RB;
-    static_assert(RB >= 1 && RB <= 8 && BLOCK >= (int)D, "one thread per digit");
+    constexpr int DPT = Sh::DPT;
+    constexpr uint32_t ROW = lb_row<RB>();
+    static_assert(RB >= 1 && RB <= 11, "digit width");
+    static_assert(DPT == 1 || DPT == 2, "at most two digits per thread (one 64-bit look-back word)");
+    static_assert(BLOCK * KPT <= 65535, "tile positions must fit the 16-bit counters");
+    uint16_t* whist = reinterpret_cast<uint16_t*>(sm.stage);
     K* s_keys = reinterpret_cast<K*>(sm.stage);
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(sm.stage);  // reused once the keys are out
 
@@ -125,9 +146,10 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
     const uint32_t wave = tid >> 6;
     constexpr bool full = FULL;  // full tiles (cnt == TILE) skip every bounds test
     const uint32_t wofs = wave * WAVE_TILE + lane;  // tile-relative index of key 0 of this lane
+    const bool owner = tid * DPT < D;              // owns digits d0 .. d0 + DPT - 1
+    const uint32_t d0 = tid * DPT;
 
-    static_assert(BLOCK * KPT <= 65535, "tile positions must fit the 16-bit counters");
-    for (uint32_t i = tid; i < WAVES * D; i += BLOCK) sm.whist[i] = 0u;
+    for (uint32_t i = tid; i < WAVES * D; i += BLOCK) whist[i] = 0u;
 
     K keys[KPT];
     uint32_t vals[KPT];
@@ -148,7 +170,7 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
         }
     }
     if (lb_next) {
-        for (uint32_t i = tid; i < LB_ROW; i += BLOCK) lb_next[tile * LB_ROW + i] = 0u;
+        for (uint32_t i = tid; i < next_row; i += BLOCK) lb_next[tile * next_row + i] = 0u;
     }
     __syncthreads();  // counters cleared
 
@@ -157,13 +179,14 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
     // observes the leader updates of iteration i.  The signal fence keeps the
     // compiler from moving the LDS accesses across iterations (it emits no
     // instruction; a volatile pointer here would turn them into flat accesses).
-    uint16_t* wh = sm.whist + wave * D;
+    uint16_t* wh = whist + wave * D;
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
-        const uint32_t word = digit_word<K>(keys[i], shift);
-        const uint32_t d = __builtin_amdgcn_ubfe(word, shift & 31u, RB);
+        uint32_t off;
+        const uint32_t word = digit_word<K>(keys[i], shift, &off);
+        const uint32_t d = __builtin_amdgcn_ubfe(word, off & 31u, RB);
         const bool valid = full || (wofs + i * 64 < cnt);
-        const uint64_t peers = match_digit<RB, FULL>(word, shift & 31u, FULL ? ~0ull : __ballot(valid));
+        const uint64_t peers = match_digit<RB, FULL>(word, off & 31u, FULL ? ~0ull : __ballot(valid));
         const uint32_t lt = mbcnt64(peers);
         const uint32_t before = wh[d];
         pos[i] = before + lt;
@@ -176,68 +199,107 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
     }
     __syncthreads();
 
-    // Per digit: exclusive prefix over waves, tile total, publish aggregate.
-    uint32_t total_d = 0;
-    if (tid < D) {
-        uint32_t run = 0;
+    // Per owned digit: exclusive prefix over waves, tile total, publish the
+    // aggregate (one 32- or 64-bit word per thread).
+    uint32_t tot[DPT];
+    uint32_t own_sum = 0;
 #pragma unroll
-        for (int w = 0; w < WAVES; ++w) {
-            const uint32_t c = sm.whist[w * D + tid];
-            sm.whist[w * D + tid] = (uint16_t)run;
-            run += c;
+    for (int j = 0; j < DPT; ++j) {
+        tot[j] = 0;
+        if (owner) {
+            uint32_t run = 0;
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) {
+                const uint32_t c = whist[w * D + d0 + j];
+                whist[w * D + d0 + j] = (uint16_t)run;
+                run += c;
+            }
+            tot[j] = run;
+            own_sum += run;
         }
-        total_d = run;
-        st_agent(&lb_cur[tile * LB_ROW + tid], (tile == first ? LB_INCLUSIVE : LB_AGGREGATE) | (total_d << 2));
+    }
+    const uint32_t flag0 = tile == first ? LB_INCLUSIVE : LB_AGGREGATE;
+    if (owner) {
+        if constexpr (DPT == 1) {
+            st_agent(&lb_cur[tile * ROW + d0], flag0 | (tot[0] << 2));
+        } else {
+            const uint64_t w = ((uint64_t)(flag0 | (tot[1] << 2)) << 32) | (flag0 | (tot[0] << 2));
+            st_agent64(reinterpret_cast<uint64_t*>(lb_cur + tile * ROW + d0), w);
+        }
     }
     uint32_t tile_total;
-    const uint32_t start = block_exclusive_scan<BLOCK>(total_d, sm.tmp, &tile_total);
-    if (tid < D) {
+    const uint32_t own_start = block_exclusive_scan<BLOCK>(own_sum, sm.tmp, &tile_total);
+    uint32_t start[DPT];
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) start[j] = own_start + (j > 0 ? tot[0] : 0u);
+    if (owner) {
         // whist[w][d] becomes the tile-local position of wave w's first digit-d key.
 #pragma unroll
-        for (int w = 0; w < WAVES; ++w) sm.whist[w * D + tid] = (uint16_t)(sm.whist[w * D + tid] + start);
+        for (int j = 0; j < DPT; ++j) {
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w)
+                whist[w * D + d0 + j] = (uint16_t)(whist[w * D + d0 + j] + start[j]);
+        }
     }
     __syncthreads();
 
-    // Stage keys in LDS in (digit, original order) order.  The digit is
-    // recomputed from the key (opaque here, so the ranking loop's counter
-    // addresses are not kept alive for this loop).
+    // Tile-local position of every key; the digit is recomputed from the key
+    // (opaque here, so the ranking loop's counter addresses are not kept alive
+    // for this loop).  The counters share LDS with the staging area, so all
+    // positions are read before any key is staged.
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
         const bool valid = full || (wofs + i * 64 < cnt);
         if (valid) {
             asm volatile("" : "+v"(keys[i]));
-            const uint32_t d = digit_of<K, RB>(keys[i], shift);
-            const uint32_t lp = sm.whist[wave * D + d] + pos[i];
-            s_keys[lp] = keys[i];
-            pos[i] = lp;
+            pos[i] += whist[wave * D + digit_of<K, RB>(keys[i], shift)];
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+        const bool valid = full || (wofs + i * 64 < cnt);
+        if (valid) s_keys[pos[i]] = keys[i];
+    }
 
-    // Decoupled look-back, one thread per digit, down to the segment's first tile
-    // (which always publishes inclusive counts, so the walk never passes it).
-    // Four predecessors are fetched per round trip: agent-scope loads bypass
-    // the XCD-local L2, so each round trip is long.  The wait is bounded: after
+    // Decoupled look-back, one thread per digit (or per pair of digits, read
+    // as one 64-bit word: both halves are always published together, so they
+    // carry the same flag), down to the segment's first tile (which always
+    // publishes inclusive counts, so the walk never passes it).  Four
+    // predecessors are fetched per round trip: agent-scope loads bypass the
+    // XCD-local L2, so each round trip is long.  The wait is bounded: after
     // spin_limit empty polls the whole workgroup counts the digits of the
     // segment's earlier keys itself (exact, just slower), so progress never
     // depends on the order in which the hardware starts workgroups.
     if (tid == 0) sm.fallback = (spin_limit == 0u && tile != first) ? 1u : 0u;  // 0: always count directly
-    uint32_t excl = 0;
+    uint32_t excl[DPT];
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) excl[j] = 0;
     __syncthreads();
-    if (tid < D && tile != first && spin_limit != 0u) {
+    if (owner && tile != first && spin_limit != 0u) {
         uint32_t k = tile - 1;
         uint32_t spins = 0;
         while (true) {
-            uint32_t v[4];
+            uint64_t v[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = (k >= first + j) ? ld_agent(&lb_cur[(k - j) * LB_ROW + tid]) : 0u;
+            for (int q = 0; q < 4; ++q) {
+                if (k >= first + q) {
+                    if constexpr (DPT == 1) v[q] = ld_agent(&lb_cur[(k - q) * ROW + d0]);
+                    else v[q] = ld_agent64(reinterpret_cast<const uint64_t*>(lb_cur + (k - q) * ROW + d0));
+                } else {
+                    v[q] = 0;
+                }
+            }
             bool found = false;
             uint32_t used = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (found || used != (uint32_t)j) break;
-                const uint32_t f = v[j] & LB_FLAG_MASK;
+            for (int q = 0; q < 4; ++q) {
+                if (found || used != (uint32_t)q) break;
+                const uint32_t lo = (uint32_t)v[q];
+                const uint32_t f = lo & LB_FLAG_MASK;
                 if (f == LB_NOT_READY) break;
-                excl += v[j] >> 2;
+                excl[0] += lo >> 2;
+                if constexpr (DPT == 2) excl[1] += (uint32_t)(v[q] >> 32) >> 2;
                 if (f == LB_INCLUSIVE) found = true;
                 else ++used;
             }
@@ -254,15 +316,29 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
     }
     __syncthreads();
     if (sm.fallback) {
-        for (uint32_t i = tid; i < D; i += BLOCK) sm.fb[i] = 0u;
+        uint32_t* fb = sm.gbase;
+        for (uint32_t i = tid; i < D; i += BLOCK) fb[i] = 0u;
         __syncthreads();
-        for (uint32_t i = seg_base + tid; i < base; i += BLOCK) atomicAdd(&sm.fb[digit_of<K, RB>(sk[i], shift)], 1u);
+        for (uint32_t i = seg_base + tid; i < base; i += BLOCK) atomicAdd(&fb[digit_of<K, RB>(sk[i], shift)], 1u);
         __syncthreads();
-        if (tid < D) excl = sm.fb[tid];
+        if (owner) {
+#pragma unroll
+            for (int j = 0; j < DPT; ++j) excl[j] = fb[d0 + j];
+        }
+        __syncthreads();  // the counts are read before gbase is written below
     }
-    if (tid < D) {
-        if (tile != first) st_agent(&lb_cur[tile * LB_ROW + tid], LB_INCLUSIVE | ((excl + total_d) << 2));
-        sm.gbase[tid] = prefix[tid] + add + excl - start;
+    if (owner) {
+        if (tile != first) {
+            if constexpr (DPT == 1) {
+                st_agent(&lb_cur[tile * ROW + d0], LB_INCLUSIVE | ((excl[0] + tot[0]) << 2));
+            } else {
+                const uint64_t w = ((uint64_t)(LB_INCLUSIVE | ((excl[1] + tot[1]) << 2)) << 32) |
+                                   (LB_INCLUSIVE | ((excl[0] + tot[0]) << 2));
+                st_agent64(reinterpret_cast<uint64_t*>(lb_cur + tile * ROW + d0), w);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) sm.gbase[d0 + j] = prefix[d0 + j] + add + excl[j] - start[j];
     }
     __syncthreads();
 
@@ -303,11 +379,13 @@ TC_DEV void onesweep_tile(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* __rest
                           const uint32_t* __restrict__ sv, uint32_t* __restrict__ dv, uint32_t base,
                           uint32_t cnt, uint32_t shift, uint32_t tile, uint32_t first, uint32_t seg_base,
                           const uint32_t* __restrict__ prefix, uint32_t add, uint32_t* __restrict__ lb_cur,
-                          uint32_t* __restrict__ lb_next, uint32_t spin_limit) {
+                          uint32_t* __restrict__ lb_next, uint32_t next_row, uint32_t spin_limit) {
     if (cnt == OnesweepShared<K, BLOCK, KPT, RB>::TILE)
         onesweep_tile_impl<K, PAIRS, BLOCK, KPT, RB, true, STREAM>(sm, sk, dk, sv, dv, base, cnt, shift, tile, first,
-                                                           seg_base, prefix, add, lb_cur, lb_next, spin_limit);
+                                                                   seg_base, prefix, add, lb_cur, lb_next, next_row,
+                                                                   spin_limit);
     else
         onesweep_tile_impl<K, PAIRS, BLOCK, KPT, RB, false, STREAM>(sm, sk, dk, sv, dv, base, cnt, shift, tile, first,
-                                                            seg_base, prefix, add, lb_cur, lb_next, spin_limit);
+                                                                    seg_base, prefix, add, lb_cur, lb_next, next_row,
+                                                                    spin_limit);
 }

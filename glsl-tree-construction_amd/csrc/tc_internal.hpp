@@ -79,9 +79,12 @@ struct tc_ctx {
     double prof_ms[K_COUNT] = {};
     uint64_t prof_n[K_COUNT] = {};
     int num_cus = 256;
-    // Sort launch parameters (used when no tuned shape applies).
+    // Sort launch parameters (used when no tuned shape applies): workgroup
+    // size, keys per lane and digit plan (8: 8-bit digits; 11: digits of at
+    // most 11 bits, fewer passes).
     int sort_block = 512;
     int sort_kpt = 20;
+    int sort_bits = 8;
     // Tuned launch shapes (tc_autotune_sort): index into SORT_SHAPES per
     // [key bytes 4 / 8][keys only / pairs][ceil(log2 n)], -1 where not measured.
     bool tune_on = false;
@@ -118,19 +121,26 @@ constexpr size_t SMALL_WORDS = 4096;
 
 // Scratch sizing helpers (bytes).
 int tc_ensure_scratch(tc_ctx* ctx, size_t bytes);
-size_t tc_sort_scratch_bytes(uint32_t n, int passes, int tile);
+// Sort scratch for n keys with any built shape and plan.
+size_t tc_sort_scratch_max_bytes(uint32_t n);
+// The sort's histogram accumulators (ctx->hist_acc).
+size_t tc_hist_acc_bytes();
 
-// Binning-pass launch shapes (workgroup size, keys per lane) the library is
-// built with; the tuner picks among them.
+// Binning-pass launch shapes (workgroup size, keys per lane, digit plan: 8 =
+// 8-bit digits, 11 = digits of at most 11 bits) the library is built with; the
+// tuner picks among them.
 struct SortShape {
-    int block, kpt;
+    int block, kpt, bits;
 };
-constexpr int N_SORT_SHAPES = 10;
-constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {{256, 8},  {512, 8},  {512, 12}, {512, 16}, {512, 20},
-                                                  {512, 24}, {512, 28}, {512, 32}, {512, 36}, {1024, 16}};
+constexpr int N_SORT_SHAPES = 12;
+constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {
+    {256, 8, 8},  {512, 8, 8},  {512, 12, 8}, {512, 16, 8}, {512, 20, 8},   {512, 24, 8},
+    {512, 28, 8}, {512, 32, 8}, {512, 36, 8}, {1024, 16, 8}, {1024, 8, 11}, {1024, 16, 11}};
 // The shape a sort of n keys uses: the tuned one for the nearest measured size
 // when the context has been tuned, else the context's fixed shape.
 SortShape tc_sort_shape(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n);
+// The same restricted to 8-bit plans (the segmented batch sort's passes).
+SortShape tc_sort_shape_8bit(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n);
 size_t tc_scan_scratch_bytes(uint32_t n);
 // Looks up this device's address of the no-alloc sort's static scratch
 // (current device = ctx->device); TC_ERR_HIP if the lookup fails.

@@ -227,6 +227,11 @@ SIGNATURES = {
     "tc_set_tuned_sort_shape": (I, [P, I, I, I, I, I]),
     "tc_clear_sort_tuning": (I, [P]),
     "tc_sort_shape_list": (I, [C.POINTER(I), C.POINTER(I), I]),
+    "tc_sort_plan_list": (I, [C.POINTER(I), C.POINTER(I), C.POINTER(I), I]),
+    "tc_set_sort_plan": (I, [P, I, I, I]),
+    "tc_get_sort_plan": (I, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+    "tc_tuned_sort_plan": (I, [P, I, I, U32, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+    "tc_set_tuned_sort_plan": (I, [P, I, I, I, I, I, I]),
     "tc_sort_keys_u32_noalloc": (I, [P, P, P, U32, S]),
     "tc_morton30": (I, [P, P, P, U32, F, F, S]),
     "tc_morton63": (I, [P, P, P, U32, F, F, S]),

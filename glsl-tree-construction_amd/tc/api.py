@@ -76,6 +76,16 @@ class Context:
     def set_sort_params(self, block: int, kpt: int):
         check(lib().tc_set_sort_params(self._h, block, kpt), "tc_set_sort_params")
 
+    def set_sort_plan(self, block: int, kpt: int, digit_bits: int = 8):
+        """Fixed sort launch shape and digit plan (8: 8-bit digits; 11: digits
+        of at most 11 bits, fewer passes)."""
+        check(lib().tc_set_sort_plan(self._h, block, kpt, digit_bits), "tc_set_sort_plan")
+
+    def sort_plan(self):
+        b, k, d = C.c_int(), C.c_int(), C.c_int()
+        check(lib().tc_get_sort_plan(self._h, C.byref(b), C.byref(k), C.byref(d)), "tc_get_sort_plan")
+        return b.value, k.value, d.value
+
     def profile(self, on: bool = True):
         check(lib().tc_profile_enable(self._h, int(bool(on))), "tc_profile_enable")
 

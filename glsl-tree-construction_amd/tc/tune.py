@@ -1,6 +1,7 @@
-"""Autotuner of the sort's launch shape (tc_autotune_sort).
+"""Autotuner of the sort's launch shape and digit plan (tc_autotune_sort).
 
-The binning pass is built for ten (workgroup size, keys per lane) shapes;
+The binning pass is built for twelve (workgroup size, keys per lane, digit
+bits) combinations: 8-bit digits, or digits of at most 11 bits (fewer passes);
 which is fastest depends on the key width, the payload and the size.
 ``autotune`` times them all on the device and the context then picks, for each
 sort, the shape measured at the nearest size (batches: at their total point
@@ -17,11 +18,12 @@ VARIANTS = {"u32_keys": (32, 0), "u32_pairs": (32, 1), "u64_keys": (64, 0), "u64
 
 
 def built_shapes() -> list:
-    """The (block, keys per thread) launch shapes the library is built with."""
-    k = lib().tc_sort_shape_list(None, None, 0)
-    b, t = (C.c_int * k)(), (C.c_int * k)()
-    lib().tc_sort_shape_list(b, t, k)
-    return list(zip(b, t))
+    """The (block, keys per thread, digit bits) launch shapes and digit plans
+    the library is built with."""
+    k = lib().tc_sort_plan_list(None, None, None, 0)
+    b, t, d = (C.c_int * k)(), (C.c_int * k)(), (C.c_int * k)()
+    lib().tc_sort_plan_list(b, t, d, k)
+    return list(zip(b, t, d))
 
 
 def autotune(ctx, max_n: int = 1 << 24) -> dict:
@@ -30,10 +32,11 @@ def autotune(ctx, max_n: int = 1 << 24) -> dict:
 
 
 def shape(ctx, key_bits: int, pairs: bool, n: int) -> tuple:
-    b, k = C.c_int(), C.c_int()
-    check(lib().tc_tuned_sort_shape(ctx.handle, key_bits, int(bool(pairs)), n, C.byref(b), C.byref(k)),
-          "tc_tuned_sort_shape")
-    return b.value, k.value
+    """(block, keys per thread, digit bits) a sort of n keys uses now."""
+    b, k, d = C.c_int(), C.c_int(), C.c_int()
+    check(lib().tc_tuned_sort_plan(ctx.handle, key_bits, int(bool(pairs)), n, C.byref(b), C.byref(k), C.byref(d)),
+          "tc_tuned_sort_plan")
+    return b.value, k.value, d.value
 
 
 def table(ctx, max_n: int = 1 << 30) -> dict:
@@ -54,9 +57,11 @@ def load(ctx, path: str):
         t = json.load(f)
     for name, entries in t.items():
         bits, pairs = VARIANTS[name]
-        for lg, (block, kpt) in entries.items():
-            check(lib().tc_set_tuned_sort_shape(ctx.handle, bits, pairs, int(lg), block, kpt),
-                  "tc_set_tuned_sort_shape")
+        for lg, entry in entries.items():
+            block, kpt = entry[0], entry[1]
+            digit_bits = entry[2] if len(entry) > 2 else 8
+            check(lib().tc_set_tuned_sort_plan(ctx.handle, bits, pairs, int(lg), block, kpt, digit_bits),
+                  "tc_set_tuned_sort_plan")
 
 
 def clear(ctx):

@@ -76,24 +76,47 @@ int tc_destroy(tc_ctx* ctx);
  * calls allocate nothing (needed before stream capture). */
 int tc_reserve(tc_ctx* ctx, uint32_t max_n);
 /* Launch parameters of the sort's binning pass, one of the compiled shapes
- * (block, keys per thread): 256 x 8, 512 x {8, 12, 16, 20, 24, 28, 32, 36} and
- * 1024 x 16.  0 keeps the current value. */
+ * (block, keys per thread) with 8-bit digits: 256 x 8, 512 x {8, 12, 16, 20,
+ * 24, 28, 32, 36} and 1024 x 16.  0 keeps the current value. */
 int tc_set_sort_params(tc_ctx* ctx, int block, int keys_per_thread);
 int tc_get_sort_params(tc_ctx* ctx, int* block, int* keys_per_thread);
+/* The same with the digit plan (no reference counterpart; the reference
+ * hard-codes 8-bit digits, shader/new_radix_sort.comp:12-16): digit_bits 8
+ * sorts in 8-bit passes (the last one narrower for 30- and 63-bit Morton
+ * codes); 11 sorts in ceil(key_bits / 11) passes of balanced widths of at most
+ * 11 bits (30-bit codes: 3 x 10; 32-bit keys: 11, 11, 10; 63-bit codes:
+ * 3 x 11 + 3 x 10; 64-bit keys: 4 x 11 + 2 x 10), built for 1024 x 8 and
+ * 1024 x 16.  A plan with an odd number of passes leaves the result in the
+ * caller's keys all the same (Morton sorts write their codes to the ping-pong
+ * buffers first; plain sorts copy back at the end). */
+int tc_set_sort_plan(tc_ctx* ctx, int block, int keys_per_thread, int digit_bits);
+int tc_get_sort_plan(tc_ctx* ctx, int* block, int* keys_per_thread, int* digit_bits);
+/* Every built (block, keys per thread, digit bits) combination, the tuner's
+ * choices: writes up to max entries (any pointer may be NULL) and returns
+ * how many there are. */
+int tc_sort_plan_list(int* blocks, int* keys_per_thread, int* digit_bits, int max);
 /* Autotuner (no reference counterpart): times every binning-pass launch shape
- * on random keys at sizes 2^12, 2^14, ... up to max_n, for u32 and u64 keys with
- * and without a u32 payload, and from then on each sort uses the fastest shape
- * measured at the size nearest its own (batches: at their total point capacity).
- * Blocking.  tc_set_sort_params switches back to one fixed shape. */
+ * and digit plan (tc_sort_plan_list) on random keys at sizes 2^12, 2^14, ... up
+ * to max_n, for u32 and u64 keys with and without a u32 payload, and from then
+ * on each sort uses the fastest measured at the size nearest its own (batches:
+ * at their total point capacity, among the 8-bit plans their segmented passes
+ * use).  Blocking.  tc_set_sort_params / tc_set_sort_plan switch back to one
+ * fixed shape. */
 int tc_autotune_sort(tc_ctx* ctx, uint32_t max_n);
 /* The shape a sort of n keys (key_bits 32 or 64, pairs 0/1) would use now. */
 int tc_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, uint32_t n, int* block, int* keys_per_thread);
-/* Restore one entry of a saved tuning table (size bucket ceil(log2 n)). */
+/* The same with the digit plan (8 or 11, see tc_set_sort_plan). */
+int tc_tuned_sort_plan(tc_ctx* ctx, int key_bits, int pairs, uint32_t n, int* block, int* keys_per_thread,
+                       int* digit_bits);
+/* Restore one entry of a saved tuning table (size bucket ceil(log2 n)); the
+ * first form restores an 8-bit plan. */
 int tc_set_tuned_sort_shape(tc_ctx* ctx, int key_bits, int pairs, int log2_n, int block, int keys_per_thread);
+int tc_set_tuned_sort_plan(tc_ctx* ctx, int key_bits, int pairs, int log2_n, int block, int keys_per_thread,
+                           int digit_bits);
 int tc_clear_sort_tuning(tc_ctx* ctx);
 /* The binning-pass launch shapes the library is built with (the tuner's
- * choices): writes up to max (block, keys per thread) pairs and returns how
- * many there are. */
+ * choices, without their digit plans; tc_sort_plan_list has those): writes up
+ * to max (block, keys per thread) pairs and returns how many there are. */
 int tc_sort_shape_list(int* blocks, int* keys_per_thread, int max);
 /* Bound on a workgroup's look-back wait (empty polls of a predecessor's
  * status) in the sort and scan kernels; past it the workgroup computes its
@@ -157,7 +180,8 @@ int tc_binning_pass_u64(tc_ctx* ctx, const uint64_t* src_k, uint64_t* dst_k, con
                         uint32_t* dst_v, uint32_t n, const uint32_t* hist, uint32_t radix_shift,
                         tc_stream stream);
 
-/* ---- Full stable LSD sorts (histogram + 4 or 8 binning passes), ascending.
+/* ---- Full stable LSD sorts (histogram + one binning pass per digit of the
+ *      digit plan: 4 or 8 with 8-bit digits), ascending.
  * Contract of u_sort.sort(), src/main.rs:85; API shape of radix_sort,
  * src/radixsort.rs:36.  The result ends in keys (and vals); alt buffers are
  * ping-pong space of n elements. */

@@ -45,6 +45,14 @@ def test_python_bindings_cover_the_header(tclib):
     assert not missing, missing
 
 
+def test_sort_plan_list(tclib):
+    """The launch shapes and digit plans the tuner chooses among (and the GPU
+    launch-shape tests cover)."""
+    from test_gpu_parity import SORT_PLANS
+
+    assert tclib.tune.built_shapes() == SORT_PLANS
+
+
 def test_status_strings(tclib):
     L = tclib.lib()
     assert L.tc_status_string(0) == b"ok"

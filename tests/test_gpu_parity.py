@@ -155,29 +155,51 @@ def test_decode_kernel(gpu_ctx, oracle):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("block,kpt", [(256, 8), (512, 8), (512, 12), (512, 16), (512, 20), (512, 24), (512, 28),
-                                       (512, 32), (512, 36), (1024, 16)])
-def test_sort_launch_shapes(gpu_ctx, oracle, block, kpt):
-    """Every binning launch shape the tuner may pick gives the stable order."""
-    r = np.random.default_rng(block * 100 + kpt)
+SORT_PLANS = [(256, 8, 8), (512, 8, 8), (512, 12, 8), (512, 16, 8), (512, 20, 8), (512, 24, 8), (512, 28, 8),
+              (512, 32, 8), (512, 36, 8), (1024, 16, 8), (1024, 8, 11), (1024, 16, 11)]
+
+
+@pytest.mark.parametrize("block,kpt,bits", SORT_PLANS)
+def test_sort_launch_shapes(gpu_ctx, oracle, tclib, block, kpt, bits):
+    """Every binning launch shape and digit plan the tuner may pick gives the
+    stable order: u32 / u64 keys with and without payload (plans with an odd
+    number of passes copy the result back), and Morton sorts, whose 30- and
+    63-bit codes get their own digit widths (3 x 10 and 3 x 11 + 3 x 10 bits
+    in the wide plans)."""
+    r = np.random.default_rng(block * 100 + kpt + bits)
     n = 300_001
     keys = r.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
     keys[::7] = keys[3]
     vals = np.arange(n, dtype=np.uint32)
-    prev = gpu_ctx.sort_params()
-    gpu_ctx.set_sort_params(block, kpt)
+    prev = gpu_ctx.sort_plan()
+    gpu_ctx.set_sort_plan(block, kpt, bits)
     try:
         k, v = dev(keys), dev(vals)
         gpu_ctx.sort(k, empty(n, np.uint32), v, empty(n, np.uint32))
         sk, sv = oracle.radix_sort(keys, vals)
         assert np.array_equal(host(k, np.uint32), sk) and np.array_equal(host(v, np.uint32), sv)
+        k = dev(keys)
+        gpu_ctx.sort(k, empty(n, np.uint32))
+        assert np.array_equal(host(k, np.uint32), sk)
         k64 = (keys.astype(np.uint64) << np.uint64(31)) | r.integers(0, 1 << 31, size=n, dtype=np.uint64)
         k, v = dev(k64), dev(vals)
         gpu_ctx.sort(k, empty(n, np.uint64), v, empty(n, np.uint32))
         sk, sv = oracle.radix_sort(k64, vals)
         assert np.array_equal(host(k, np.uint64), sk) and np.array_equal(host(v, np.uint32), sv)
+        pts = tclib.gen_gaussian_mixture(n, block + kpt + bits, 8, 3.0)
+        d = dev(pts)
+        for cb, kdt in ((30, np.uint32), (63, np.uint64)):
+            codes = oracle.compute_morton(pts, 0.0, 1024.0) if cb == 30 else oracle.compute_morton63(pts, 0.0, 1024.0)
+            want_k, want_v = oracle.radix_sort(codes, vals)
+            mk, mv, ka, va = empty(n, kdt), empty(n, np.uint32), empty(n, kdt), empty(n, np.uint32)
+            gpu_ctx.morton_sort_pairs(d, mk, mv, ka, va, 0.0, 1024.0)
+            assert np.array_equal(host(mk, kdt, n), want_k), cb
+            assert np.array_equal(host(mv, np.uint32, n), want_v), cb
+            mk = empty(n, kdt)
+            gpu_ctx.morton_sort(d, mk, empty(n, kdt), 0.0, 1024.0)
+            assert np.array_equal(host(mk, kdt, n), want_k), cb
     finally:
-        gpu_ctx.set_sort_params(*prev)
+        gpu_ctx.set_sort_plan(*prev)
 
 
 @pytest.mark.parametrize("n", [1, 2, 1000, 300_001, 1 << 20])

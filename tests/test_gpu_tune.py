@@ -34,11 +34,13 @@ def test_autotune_table_and_sorts(oracle, tclib, tmp_path):
         path = str(tmp_path / "tune.json")
         tc.tune.save(ctx, path, 1 << 20)
         tc.tune.clear(ctx)
-        assert tc.tune.shape(ctx, 32, False, 1 << 20) == ctx.sort_params()
+        assert tc.tune.shape(ctx, 32, False, 1 << 20) == ctx.sort_plan()
         tc.tune.load(ctx, path)
         assert tc.tune.table(ctx, 1 << 20) == t
         # An explicit shape replaces the tuned ones.
         ctx.set_sort_params(512, 28)
-        assert tc.tune.shape(ctx, 64, True, 1 << 16) == (512, 28)
+        assert tc.tune.shape(ctx, 64, True, 1 << 16) == (512, 28, 8)
+        ctx.set_sort_plan(1024, 16, 11)
+        assert tc.tune.shape(ctx, 32, False, 1 << 16) == (1024, 16, 11)
     finally:
         ctx.close()
