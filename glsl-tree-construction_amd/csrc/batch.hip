@@ -70,10 +70,11 @@ namespace {
 constexpr int SEG_BLOCK = 256;
 // Workgroups per CU of the segmented tree kernels (SEG_BLOCK = one wave per
 // SIMD, so also waves per SIMD): their launch bounds ask the compiler for this
-// occupancy and their grids hold four times this many per CU.  The octree kernel
-// fits 5 without spills (96 VGPRs); at 8 (27 registers spilled) it is slower.
+// occupancy and their grids hold four times this many per CU.  Both fit 8
+// without spills since the window staging keeps its per-lane load addresses
+// out of the tile loops (the octree kernel needed 111 VGPRs before).
 constexpr int SEG_BRT_PER_CU = 8;
-constexpr int SEG_OCT_PER_CU = 5;
+constexpr int SEG_OCT_PER_CU = 8;
 constexpr int SEG_ITEMS = 8;  // point loads in flight per lane in the Morton kernel
 constexpr int SEG_BRT_TILE = 1024;  // slots per workgroup step of the tree kernels
 constexpr int SEG_BRT_HALO = 512;   // LDS window halo of the octree walk

@@ -38,7 +38,11 @@
 namespace {
 
 constexpr int HIST_BLOCK = 256;
-constexpr int HIST_REP = 4;  // LDS sub-histogram copies per block (one per wave) of the 8-bit plans
+// LDS sub-histogram copies per block of the 8-bit plans.  One copy shared by the
+// block's four waves measured faster than one per wave (2^26 keys, four 8-bit
+// digits: 0.0895 against 0.1006 ms on Morton codes, 0.0754 against 0.0810 on
+// random keys): less LDS per block, more blocks resident.
+constexpr int HIST_REP = 1;
 constexpr int HG_ITEMS = 4;  // 16-byte key loads in flight per thread in the histogram kernel
 constexpr uint32_t HSTRIDE = 2048;  // words per pass in histogram tables and accumulators (digits <= 11 bits)
 constexpr uint32_t ACC_STRIDE = 8 * HSTRIDE;  // words per accumulator copy (8 passes)
@@ -93,8 +97,7 @@ constexpr int wide_passes() {
     return sizeof(K) == 4 ? 3 : 6;
 }
 
-// LDS histograms of one block.  8-bit plans: HIST_REP copies (one per wave) of
-// DIG x 256 bins.  Wide plans: one copy, pass p's 2^width[p] bins at
+// LDS histograms of one block.  8-bit plans: HIST_REP copies of DIG x 256 bins.  Wide plans: one copy, pass p's 2^width[p] bins at
 // offset sum_{q<p} 2^width[q] (at most 11, 11, 10 for u32 keys and
 // 11, 11, 11, 11, 10, 10 for u64 keys).
 template <typename K, bool WIDE>
@@ -432,10 +435,9 @@ int dispatch_pass(tc_ctx* ctx, SortShape sh, uint32_t tiles, hipStream_t s, cons
     return TC_ERR_INVALID;
 }
 
-// Four times as many workgroups as are resident at once (eight per CU for
-// u32 keys, five for u64 keys, whose 32 KiB of LDS sub-histograms fit five per
-// CU): each streams a few chunks and the hardware hands the next workgroup to
-// whichever CU frees up first.  Measured on the fused Morton + histogram
+// Several times as many workgroups as are resident at once (32 per CU for u32
+// keys, 20 for u64 keys): each streams a few chunks and the hardware hands the
+// next workgroup to whichever CU frees up first.  Measured on the fused Morton + histogram
 // kernel at 2^26 points: 0.248 ms with 8 workgroups per CU, 0.240 with 16,
 // 0.233 with 32.  The merge's atomic chains stay gridDim / HIST_COPIES long.
 template <typename K>

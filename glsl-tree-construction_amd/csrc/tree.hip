@@ -100,9 +100,9 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_edge_count(const uint8_t* __rest
 }
 
 template <typename K>
-// Eight workgroups per CU resident (64 VGPRs, 14 spilled): at its natural 79
-// VGPRs only 6 fit, and with a grid of 8 per CU the remaining quarter ran after
-// the rest (measured: 1.78 -> 1.52 ms on 65 M radix nodes).
+// Eight workgroups per CU resident (61 VGPRs, nothing spilled): with fewer,
+// a grid of 8 per CU left the remaining workgroups to run after the rest
+// (measured at 6 per CU: 1.78 against 1.52 ms on 65 M radix nodes).
 __global__ __launch_bounds__(TREE_BLOCK, 8) void k_octree_build(
     tc_oct_node* __restrict__ nodes, uint32_t node_cap, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ ec, const K* __restrict__ codes, const uint8_t* __restrict__ prefix_n,

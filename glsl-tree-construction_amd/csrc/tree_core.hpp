@@ -73,28 +73,35 @@ struct LinkView {
 // load-then-store loop waits once per iteration).  Indexes are clamped into
 // the valid range (which must not be empty) instead of branching around the
 // loads; entries outside [lo, hi) then get `fill`.
+// The lane index is opaque for the reason given at stage_links.
 template <int BLOCK, int W, typename K>
 TC_DEV void stage_codes(K* dst, const K* __restrict__ src, int64_t j0, int64_t lo, int64_t hi, K fill) {
     constexpr int PER = (W + BLOCK - 1) / BLOCK;
     K v[PER];
+    uint32_t lane_t = threadIdx.x;
+    asm volatile("" : "+v"(lane_t));
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-        const int64_t j = j0 + r * BLOCK + (int)threadIdx.x;
+        const int64_t j = j0 + r * BLOCK + (int)lane_t;
         v[r] = src[j < lo ? lo : (j < hi ? j : hi - 1)];
     }
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-        const int64_t j = j0 + r * BLOCK + (int)threadIdx.x;
+        const int64_t j = j0 + r * BLOCK + (int)lane_t;
         if (j < lo || j >= hi) v[r] = fill;
     }
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-        const int q = r * BLOCK + (int)threadIdx.x;
+        const int q = r * BLOCK + (int)lane_t;
         if (q < W) dst[q] = v[r];
     }
 }
 
-// Links of entries [w0, w1) (0 < w1 - w0 <= W) packed for LinkView.
+// Links of entries [w0, w1) (0 < w1 - w0 <= W) packed for LinkView.  The
+// lane index is made opaque here: otherwise the compiler hoists the per-lane
+// part of all 4 x PER load addresses (64-bit each) out of the caller's tile
+// loop, keeps them live across it and spills them (14 VGPRs of the octree
+// kernel at eight workgroups per CU, reloaded from scratch every tile).
 template <int BLOCK, int W>
 TC_DEV void stage_links(int32_t* s_left, uint16_t* s_meta, const int32_t* __restrict__ left,
                         const uint32_t* __restrict__ ec, const uint8_t* __restrict__ hll,
@@ -103,9 +110,11 @@ TC_DEV void stage_links(int32_t* s_left, uint16_t* s_meta, const int32_t* __rest
     int32_t vl[PER];
     uint32_t ve[PER];
     uint8_t va[PER], vb[PER];
+    uint32_t lane_t = threadIdx.x;
+    asm volatile("" : "+v"(lane_t));
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-        int64_t j = w0 + r * BLOCK + (int)threadIdx.x;
+        int64_t j = w0 + r * BLOCK + (int)lane_t;
         j = j < w1 ? j : w1 - 1;
         vl[r] = left[j];
         ve[r] = ec[j];
@@ -114,7 +123,7 @@ TC_DEV void stage_links(int32_t* s_left, uint16_t* s_meta, const int32_t* __rest
     }
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-        const int q = r * BLOCK + (int)threadIdx.x;
+        const int q = r * BLOCK + (int)lane_t;
         if (w0 + q < w1) {
             s_left[q] = vl[r];
             s_meta[q] = LinkView::pack(ve[r], va[r], vb[r]);
@@ -354,19 +363,18 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
     const K code = codes[gb + a];
     const int bottom = prefix_n[gb + a] / 3;
 
-    // Walk (LDS reads only while inside the window).  kind: 0 none,
-    // 1 radix leaf, 2 internal node with its own chain.
+    // Walk (LDS reads only while inside the window).  Candidate p's kind is
+    // two bits of `kinds`: 0 none, 1 radix leaf, 2 internal node with its own
+    // chain (one register instead of eight keeps the kernel unspilled at
+    // eight workgroups per CU).
     uint32_t cand[8];
-    uint32_t kind[8];
+    uint32_t kinds = 0;
     uint32_t mid[2];
     bool mid_ok[2] = {false, false};
     uint32_t low[4];
     bool low_ok[4] = {false, false, false, false};
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-        cand[p] = 0;
-        kind[p] = 0;
-    }
+    for (int p = 0; p < 8; ++p) cand[p] = 0;
     auto visit = [&](uint32_t x, int pos, int stride, uint32_t* next, bool* next_ok) {
         const uint32_t lc = (uint32_t)lv.left_child(x);
         const uint32_t m = lv.meta(x);
@@ -377,12 +385,12 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
             if (c > nb) continue;  // a valid tree's children are in [0, nb]
             if ((m >> (8 + side)) & 1u) {
                 cand[ps] = c;
-                kind[ps] = link_leaves ? 1u : 0u;
+                kinds |= (link_leaves ? 1u : 0u) << (2 * ps);
             } else if (c < nb) {
                 const uint32_t e = lv.meta(c) & 255u;
                 if (e > 0 || !next) {
                     cand[ps] = c;
-                    kind[ps] = e > 0 ? 2u : 0u;
+                    kinds |= (e > 0 ? 2u : 0u) << (2 * ps);
                 } else {
                     next[side] = c;
                     next_ok[side] = true;
@@ -400,24 +408,34 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
 
     // Children of the bottom node: slot from the child's code, link = leaf
     // index or the child's chain top.
+    // Two groups of four candidates: the compiler barrier between them keeps
+    // at most four candidates' code and offset loads in flight, which bounds
+    // the kernel's registers (81 VGPRs with all eight hoisted together).
     int32_t bch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int32_t bmask = 0;
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-        if (kind[p] == 0) continue;
-        const uint32_t c = cand[p];
-        const uint32_t slot = child_slot<K>(codes[gb + c], bottom + 1);
-        int32_t v = (int32_t)c;
-        if (kind[p] == 2) {
-            v = (int32_t)(offsets[gb + c] + (lv.meta(c) & 255u) - 1 - node_base);
-            bmask |= 1 << slot;
-        }
+    for (int g = 0; g < 2; ++g) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) bch[q] = (slot == (uint32_t)q) ? v : bch[q];
+        for (int p = 4 * g; p < 4 * g + 4; ++p) {
+            const uint32_t kind = (kinds >> (2 * p)) & 3u;
+            if (kind == 0) continue;
+            const uint32_t c = cand[p];
+            const uint32_t slot = child_slot<K>(codes[gb + c], bottom + 1);
+            int32_t v = (int32_t)c;
+            if (kind == 2) {
+                v = (int32_t)(offsets[gb + c] + (lv.meta(c) & 255u) - 1 - node_base);
+                bmask |= 1 << slot;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) bch[q] = (slot == (uint32_t)q) ? v : bch[q];
+        }
+        asm volatile("" ::: "memory");
     }
 
     // Node idx (global) is stored at nodes[idx - write_base] (nodes may be one
-    // cloud's own buffer, of node_cap entries).
+    // cloud's own buffer, of node_cap entries).  The bottom node (j = 0) takes
+    // the gathered children; every chain node above has one child, the node
+    // one level deeper, whose slot and link are formed at the store.
     for (uint32_t j = 0; j < cnt; ++j) {
         const uint32_t idx = off + j;
         if (idx - write_base >= node_cap) break;
@@ -427,23 +445,18 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
         CodeTraits<K>::decode((code >> sh) << sh, min_coord, range, c4);
         // range / 2^(level - root_level), exact: a power-of-two scale.
         const float cell = ldexpf(range, root_level - level);
-        int32_t ch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        int32_t mask = 0;
+        int4* dst = reinterpret_cast<int4*>(nodes + (idx - write_base));
         if (j == 0) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) ch[q] = bch[q];
-            mask = bmask;
-        } else {  // chain-internal child: the node one level deeper
+            dst[0] = make_int4(bch[0], bch[1], bch[2], bch[3]);
+            dst[1] = make_int4(bch[4], bch[5], bch[6], bch[7]);
+            dst[3] = make_int4(__float_as_int(cell), bmask, 0, 0);
+        } else {
             const uint32_t slot = child_slot<K>(code, level + 1);
             const int32_t v = (int32_t)(idx - 1 - node_base);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) ch[q] = (slot == (uint32_t)q) ? v : 0;
-            mask = 1 << slot;
+            dst[0] = make_int4(slot == 0 ? v : 0, slot == 1 ? v : 0, slot == 2 ? v : 0, slot == 3 ? v : 0);
+            dst[1] = make_int4(slot == 4 ? v : 0, slot == 5 ? v : 0, slot == 6 ? v : 0, slot == 7 ? v : 0);
+            dst[3] = make_int4(__float_as_int(cell), 1 << slot, 0, 0);
         }
-        int4* dst = reinterpret_cast<int4*>(nodes + (idx - write_base));
-        dst[0] = make_int4(ch[0], ch[1], ch[2], ch[3]);
-        dst[1] = make_int4(ch[4], ch[5], ch[6], ch[7]);
         dst[2] = make_int4(__float_as_int(c4[0]), __float_as_int(c4[1]), __float_as_int(c4[2]), __float_as_int(c4[3]));
-        dst[3] = make_int4(__float_as_int(cell), mask, 0, 0);
     }
 }
