@@ -54,8 +54,12 @@ def measured_traffic():
 
 
 def traffic_probe(ctx, pts_h):
-    """Three headline builds, then reference kernels of known traffic on a
-    256 MiB int32 tensor: fill_ writes 256 MiB, sum reads 256 MiB."""
+    """Three headline builds, then reference kernels of known traffic:
+    tc_traffic_reference on a 1 GiB buffer (16-byte plain loads, the same
+    non-temporal -- the form the Morton, unique and binning kernels use for data
+    read once --, and 16-byte stores), and torch's fill_ / sum on 256 MiB."""
+    import ctypes as C
+
     import torch
 
     import tc
@@ -65,6 +69,14 @@ def traffic_probe(ctx, pts_h):
     tree = tc.Tree(ctx, n, 30)
     for _ in range(3):
         tree.build(pts, n, 0.0, 1024.0)
+    L = tc.lib()
+    big = torch.ones(1 << 28, dtype=torch.int32, device="cuda")  # 1 GiB
+    sink = torch.zeros(2048, dtype=torch.int32, device="cuda")
+    for mode in (0, 1, 2):
+        for _ in range(3):
+            st = L.tc_traffic_reference(ctx.handle, C.c_void_p(big.data_ptr()), big.numel() * 4, mode,
+                                        C.c_void_p(sink.data_ptr()), None)
+            assert st == 0, st
     a = torch.empty(1 << 26, dtype=torch.int32, device="cuda")
     for _ in range(3):
         a.fill_(1)
@@ -118,40 +130,68 @@ def profile_stages(ctx, fn, reps):
     return {k: (ms / cnt, cnt // reps) for k, (ms, cnt) in rep.items()}
 
 
-def run_cpu_baseline(pts, reps=2):
-    """The CPU port of the pipeline (the oracle) on this host: multi-threaded on
-    the host's CPU share (at most 16 threads; the value reported), single-
-    threaded, and std::sort of the codes (the reference's only CPU stage)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """Threads for the multi-threaded baseline: the cores this process may run
+    on, capped at the CPU share of a one-GPU box (16)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = int(os.environ.get("OMP_NUM_THREADS") or avail)
+    return max(1, min(avail, env, 16))
+
+
+def run_cpu_baseline(pts, reps=5):
+    """The CPU port of the pipeline (the oracle, gcc/g++ -O3) on this host, timed
+    per stage: one warm-up, then the median of `reps` runs per stage; multi-
+    threaded on the host's cores (the value reported), single-threaded, and
+    std::sort of the codes (the reference's only CPU stage)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    import numpy as np
 
-    threads = min(int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1), os.cpu_count() or 1, 16)
+    threads = host_threads()
     n = len(pts)
-    mt, stages = [], {}
-    for _ in range(reps + 1):
+    runs = []
+    for r in range(reps + 1):
         st = {}
         oracle.build_tree_mt(pts, 0.0, 1024.0, 30, threads=threads, timings=st)
-        mt.append(sum(st.values()) * 1e-3)  # the stages only, as the GPU build (output arrays preallocated)
-        if mt[-1] == min(mt):
-            stages = st
+        if r:
+            runs.append(st)  # the stages only, as the GPU build (output arrays preallocated)
+    stages = {k: float(np.median([x[k] for x in runs])) for k in runs[0]}
+    t_mt = sum(stages.values()) * 1e-3
     st1 = []
-    for _ in range(reps):
+    for r in range(reps + 1):
         t0 = time.perf_counter()
         oracle.build_tree(pts, 0.0, 1024.0, 30)
-        st1.append(time.perf_counter() - t0)
-    sort_ms = oracle.std_sort_ms(oracle.compute_morton(pts, 0.0, 1024.0))
-    t_mt, t_st = min(mt), min(st1)
+        if r:
+            st1.append(time.perf_counter() - t0)
+    t_st = float(np.median(st1))
+    codes = oracle.compute_morton(pts, 0.0, 1024.0)
+    sort_ms = float(np.median([oracle.std_sort_ms(codes) for _ in range(3)]))
     return {
         "value": round(n / t_mt / 1e6, 3),
         "unit": "Mpoints/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"full workload ({n} points): the C port of the pipeline (the oracle's stages, gcc/g++ -O2/-O3) "
-                  f"on {threads} host threads, best of {reps + 1}, {t_mt:.3f} s per build (sum of the stages; "
-                  f"output arrays allocated beforehand, as for the GPU build)",
+        "cpu": cpu_model(),
+        "sample": f"full workload ({n} points): the C port of the pipeline (the oracle's stages, gcc/g++ -O3) on "
+                  f"{threads} host threads; per stage one warm-up then the median of {reps} runs, summed: "
+                  f"{t_mt:.3f} s per build (output arrays allocated beforehand, as for the GPU build)",
         "stage_ms": {k: round(v, 2) for k, v in stages.items()},
         "single_thread_Mpoints_per_s": round(n / t_st / 1e6, 3),
         "single_thread_s": round(t_st, 3),
+        "single_thread_method": f"whole oracle build, median of {reps} after one warm-up",
         "std_sort_ms": round(sort_ms, 1),
     }
 
@@ -255,22 +295,17 @@ def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04, plain=True):
         return {"total_points": total, "batched_ms": round(batched_s * 1e3, 3),
                 "batched_Mpoints_per_s": round(total / batched_s / 1e6, 1)}
 
-    # Plain: one tc_build_tree per cloud, one cloud at a time.
-    tree = tc.Tree(ctx, int(sizes.max()), 30)
+    # Plain: one tc_build_tree per cloud, one cloud at a time, from the native
+    # loop (tc_plain_run_trace with every cloud arriving at once: each build is
+    # queued and waited for in C++, no Python between clouds).
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     views = [pts[offs[i]:offs[i + 1]] for i in range(n_clouds)]
-    plain_lat = []
-    for rep in range(max(1, steps // 4)):
-        t0 = time.perf_counter()
-        per_cloud = []
-        for i in range(n_clouds):
-            c0 = time.perf_counter()
-            tree.build(views[i], int(sizes[i]), 0.0, 1024.0, stream=stream)
-            stream.synchronize()
-            per_cloud.append(time.perf_counter() - c0)
-        plain_lat.append(time.perf_counter() - t0)
-    plain_s = float(np.mean(plain_lat))
-    tree.free()
+    plain_lat, per_cloud = [], []
+    for rep in range(max(2, steps // 4)):
+        done = tc.plain_run_trace(ctx, np.zeros(n_clouds), views, sizes)
+        plain_lat.append(float(done.max()))
+        per_cloud = np.diff(np.concatenate([[0.0], np.sort(done)]))  # each build's own time
+    plain_s = float(np.median(plain_lat))
     batch.close()
     return {
         "workload": f"{n_clouds} clouds, sizes log-uniform in [2^10, 2^20] (total {total} points), alternating "
@@ -279,6 +314,7 @@ def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04, plain=True):
         "batched_ms": round(batched_s * 1e3, 3),
         "batched_Mpoints_per_s": round(total / batched_s / 1e6, 1),
         "batched_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "plain_loop": "tc_plain_run_trace: one tc_build_tree per cloud, each waited for, from C++",
         "plain_loop_ms": round(plain_s * 1e3, 3),
         "plain_Mpoints_per_s": round(total / plain_s / 1e6, 1),
         "plain_cloud_p50_ms": round(float(np.percentile(per_cloud, 50)) * 1e3, 4),
@@ -333,16 +369,53 @@ def bench_served(ctx, rates=(1000, 2000, 5000, 7500, 10000, 12000, 14000, 16000,
                       "p99_ms": round(float(np.percentile(lat_p, 99)) * 1e3, 3),
                       "Mpoints_per_s": round(total / span_p / 1e6, 1)},
         })
+    def trial(kind, lam):
+        k = int(min(max_req, lam * window_s))
+        t = np.cumsum(rng.exponential(1.0 / lam, k))
+        t -= t[0]
+        idx = rng.integers(0, pool, k)
+        req_n = sizes[idx]
+        lat = srv.run_trace(t, [pts[i] for i in idx], req_n) if kind == "served" else \
+            tc.plain_run_trace(ctx, t, [pts[i] for i in idx], req_n)
+        return float(np.percentile(lat, 99)) * 1e3, float(req_n.sum()) / float(np.max(t + lat)) / 1e6
+
+    def bisect(kind, steps=6):
+        """Highest rate whose p99 meets the SLO, between the last grid rate that
+        meets it and the next one that does not; returns (rate, Mpoints/s, p99)."""
+        grid = out["rates"]
+        ok = [r for r in grid if r[kind]["p99_ms"] <= slo_ms]
+        if not ok:
+            return None
+        lo = max(ok, key=lambda r: r["rate_req_s"])
+        above = [r for r in grid if r["rate_req_s"] > lo["rate_req_s"]]
+        best = (lo["rate_req_s"], lo[kind]["Mpoints_per_s"], lo[kind]["p99_ms"])
+        if not above:
+            return best
+        lo_r, hi_r = float(lo["rate_req_s"]), float(min(r["rate_req_s"] for r in above))
+        for _ in range(steps):
+            mid = 0.5 * (lo_r + hi_r)
+            p99, mpts = trial(kind, mid)
+            if p99 <= slo_ms:
+                lo_r = mid
+                if mpts > best[1]:
+                    best = (mid, mpts, p99)
+            else:
+                hi_r = mid
+        return best
+
+    cap = {}
+    for kind in ("served", "plain"):
+        b = bisect(kind)
+        cap[kind] = b
+        out[f"{kind}_capacity_at_slo"] = None if b is None else {
+            "rate_req_s": round(b[0], 1), "Mpoints_per_s": round(b[1], 1), "p99_ms": round(b[2], 3)}
     srv.close()
-
-    def capacity(kind):
-        ok = [r[kind]["Mpoints_per_s"] for r in out["rates"] if r[kind]["p99_ms"] <= slo_ms]
-        return max(ok) if ok else 0.0
-
-    out["served_Mpoints_per_s_at_slo"] = capacity("served")
-    out["plain_Mpoints_per_s_at_slo"] = capacity("plain")
+    out["served_Mpoints_per_s_at_slo"] = round(cap["served"][1], 1) if cap["served"] else 0.0
+    out["plain_Mpoints_per_s_at_slo"] = round(cap["plain"][1], 1) if cap["plain"] else 0.0
     out["speedup_at_slo"] = round(out["served_Mpoints_per_s_at_slo"] / out["plain_Mpoints_per_s_at_slo"], 2) \
         if out["plain_Mpoints_per_s_at_slo"] else None
+    out["capacity_method"] = ("highest rate with p99 <= SLO: the rate grid, then 6 bisection steps between the "
+                              "last grid rate that meets the SLO and the next that does not (new Poisson traces)")
     return out
 
 

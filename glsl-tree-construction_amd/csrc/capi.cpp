@@ -25,6 +25,39 @@ __global__ __launch_bounds__(FILL_BLOCK) void k_fill_u32(uint32_t* __restrict__ 
     for (size_t i = (size_t)blockIdx.x * FILL_BLOCK + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// Reference kernels of known traffic for calibrating hardware counters
+// (tc_traffic_reference): stream `nvec` 16-byte vectors through plain or
+// non-temporal loads (folded into one word per workgroup, written to sink),
+// or write them with 16-byte stores.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_traffic_read(const uint4* __restrict__ src, size_t nvec,
+                                                      uint32_t* __restrict__ sink) {
+    __shared__ uint32_t s_x[4];
+    uint32_t x = 0;
+    const size_t stride = (size_t)gridDim.x * 256;
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < nvec; i += 4 * stride) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = NT ? ld_stream(&src[i + u * stride]) : src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    for (; i < nvec; i += stride) {
+        const uint4 v = NT ? ld_stream(&src[i]) : src[i];
+        x ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    for (int off = 32; off >= 1; off >>= 1) x ^= __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63u) == 0) s_x[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) sink[blockIdx.x] = s_x[0] ^ s_x[1] ^ s_x[2] ^ s_x[3];
+}
+
+__global__ __launch_bounds__(256) void k_traffic_write(uint4* __restrict__ dst, size_t nvec, uint32_t v) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nvec; i += stride) dst[i] = make_uint4(v, v, v, v);
+}
+
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() {
@@ -40,6 +73,26 @@ struct DevBuf {
 int set_device(tc_ctx* ctx) { return hipSetDevice(ctx->device) == hipSuccess ? TC_OK : TC_ERR_HIP; }
 
 }  // namespace
+
+extern "C" int tc_traffic_reference(tc_ctx* ctx, void* buf, size_t bytes, int mode, uint32_t* sink,
+                                    tc_stream stream) {
+    if (!ctx || !buf || (mode != 2 && !sink) || mode < 0 || mode > 2) return TC_ERR_INVALID;
+    if ((reinterpret_cast<uintptr_t>(buf) & 15u) || (bytes & 15u)) return TC_ERR_INVALID;
+    const size_t nvec = bytes / 16;
+    uint32_t grid = (uint32_t)ctx->num_cus * 8;
+    if (grid > TC_TRAFFIC_SINK_WORDS) grid = TC_TRAFFIC_SINK_WORDS;  // one sink word per workgroup
+    hipStream_t s = as_stream(stream);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_traffic_read<false>, dim3(grid), dim3(256), 0, s, static_cast<const uint4*>(buf), nvec,
+                           sink);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_traffic_read<true>, dim3(grid), dim3(256), 0, s, static_cast<const uint4*>(buf), nvec,
+                           sink);
+    else
+        hipLaunchKernelGGL(k_traffic_write, dim3(grid), dim3(256), 0, s, static_cast<uint4*>(buf), nvec, 0x5A5A5A5Au);
+    TC_LAUNCH_CHECK();
+    return TC_OK;
+}
 
 int tc_fill_u32(tc_ctx* ctx, uint32_t* ptr, uint32_t value, size_t count, hipStream_t s) {
     if (count == 0) return TC_OK;

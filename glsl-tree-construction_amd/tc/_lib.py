@@ -233,6 +233,7 @@ SIGNATURES = {
     "tc_tuned_sort_plan": (I, [P, I, I, U32, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
     "tc_set_tuned_sort_plan": (I, [P, I, I, I, I, I, I]),
     "tc_sort_keys_u32_noalloc": (I, [P, P, P, U32, S]),
+    "tc_traffic_reference": (I, [P, P, C.c_size_t, I, P, S]),
     "tc_morton30": (I, [P, P, P, U32, F, F, S]),
     "tc_morton63": (I, [P, P, P, U32, F, F, S]),
     "tc_morton30_decode": (I, [P, P, P, U32, F, F, S]),
