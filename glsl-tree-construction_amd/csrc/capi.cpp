@@ -138,6 +138,7 @@ extern "C" const char* tc_status_string(int status) {
         case TC_ERR_NO_DEVICE: return "no usable GPU";
         case TC_ERR_BAD_TREE: return "tree invariant violated";
         case TC_ERR_IO: return "file I/O error";
+        case TC_ERR_CANCELLED: return "cancelled";
         default: return "unknown status";
     }
 }
@@ -300,60 +301,6 @@ extern "C" int tc_get_sort_plan(tc_ctx* ctx, int* block, int* kpt, int* digit_bi
     *kpt = ctx->sort_kpt;
     *digit_bits = ctx->sort_bits;
     return TC_OK;
-}
-
-extern "C" uint64_t tc_octree_node_bound(uint64_t n_keys, int code_bits) {
-    if (n_keys < 2) return 0;
-    const uint64_t half = n_keys / 2;
-    uint64_t total = 1, p = 1;
-    const int levels = (code_bits - 1) / 3;
-    for (int L = 1; L <= levels; ++L) {
-        p = p < (1ull << 60) ? p * 8 : p;
-        total += std::min(p, half);
-    }
-    return total;
-}
-
-// --------------------------------------------------------------- host decode
-namespace {
-uint32_t h_compact3_10(uint32_t m) {
-    uint32_t x = m & 0x09249249u;
-    x = (x ^ (x >> 2)) & 0x030c30c3u;
-    x = (x ^ (x >> 4)) & 0x0300f00fu;
-    x = (x ^ (x >> 8)) & 0x030000ffu;
-    x = (x ^ (x >> 16)) & 0x000003ffu;
-    return x;
-}
-uint64_t h_compact3_21(uint64_t m) {
-    uint64_t x = m & 0x1249249249249249ull;
-    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
-    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
-    x = (x ^ (x >> 8)) & 0x001f0000ff0000ffull;
-    x = (x ^ (x >> 16)) & 0x001f00000000ffffull;
-    x = (x ^ (x >> 32)) & 0x00000000001fffffull;
-    return x;
-}
-}  // namespace
-
-// morton32_to_xyz, src/morton.rs:43-59 (volatile keeps each fp32 step rounded).
-extern "C" void tc_morton32_to_xyz(float ret[4], uint32_t code, float min_coord, float range) {
-    for (int a = 0; a < 3; ++a) {
-        volatile float f = (float)h_compact3_10(code >> a);
-        f = f / 1024.0f;
-        f = f * range;
-        ret[a] = f + min_coord;
-    }
-    ret[3] = 1.0f;
-}
-
-extern "C" void tc_morton64_to_xyz(float ret[4], uint64_t code, float min_coord, float range) {
-    for (int a = 0; a < 3; ++a) {
-        volatile float f = (float)h_compact3_21(code >> a);
-        f = f / 2097152.0f;
-        f = f * range;
-        ret[a] = f + min_coord;
-    }
-    ret[3] = 1.0f;
 }
 
 // ------------------------------------------------------------ plain pipeline

@@ -20,6 +20,7 @@ struct SchedLimits {
     uint32_t max_clouds;  // clouds per batch
     uint32_t max_points;  // points per batch
     uint64_t budget;      // bytes of results that may be reserved at once
+    uint32_t max_queued = 0;  // requests waiting for launch (0: unbounded)
 };
 
 // First come, first served: a batch is the longest prefix of the waiting queue
@@ -30,7 +31,24 @@ class Scheduler {
   public:
     explicit Scheduler(SchedLimits lim) : lim_(lim) {}
 
-    void push(const SchedReq& r) { q_.push_back(r); }
+    // Queues a request; false (and nothing queued) when max_queued requests
+    // are already waiting: the caller reports back-pressure.
+    bool push(const SchedReq& r) {
+        if (lim_.max_queued && q_.size() >= lim_.max_queued) return false;
+        q_.push_back(r);
+        return true;
+    }
+
+    // Removes a waiting request (cancellation); false if it is not queued.
+    bool remove(uint64_t id) {
+        for (auto it = q_.begin(); it != q_.end(); ++it) {
+            if (it->id == id) {
+                q_.erase(it);
+                return true;
+            }
+        }
+        return false;
+    }
 
     // Appends the next batch to out (cleared first).  can_place(r) may refuse
     // a request (its memory could not be placed): the batch then ends there.

@@ -84,7 +84,8 @@ struct tc_server {
     std::vector<tc_server_result> finished;
     std::unordered_map<uint64_t, Held> held;
     // Statistics.
-    uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, clouds_launched = 0, points_done = 0;
+    uint64_t submitted = 0, completed = 0, rejected = 0, cancelled = 0, batches = 0, clouds_launched = 0,
+             points_done = 0;
     std::vector<double> lat_ms;
     double first_submit = -1.0, last_done = 0.0;
 
@@ -270,6 +271,7 @@ extern "C" void tc_server_config_default(tc_server_config* cfg) {
     cfg->max_batch_points = 1u << 24;
     cfg->code_bits = 30;
     cfg->link_leaves = 1;
+    cfg->max_queued = 1u << 16;
 }
 
 extern "C" int tc_server_create(tc_ctx* ctx, const tc_server_config* cfg_in, tc_server** out) {
@@ -293,7 +295,7 @@ extern "C" int tc_server_create(tc_ctx* ctx, const tc_server_config* cfg_in, tc_
     s->cfg = cfg;
     s->device = dev;
     s->t0 = Clock::now();
-    s->sched = Scheduler(SchedLimits{cfg.max_batch_clouds, cfg.max_batch_points, arena_bytes});
+    s->sched = Scheduler(SchedLimits{cfg.max_batch_clouds, cfg.max_batch_points, arena_bytes, cfg.max_queued});
     int st = TC_OK;
     if (hipMalloc((void**)&s->arena_base, arena_bytes) != hipSuccess) st = TC_ERR_NOMEM;
     s->arena.init(arena_bytes);
@@ -364,15 +366,40 @@ extern "C" int tc_server_submit(tc_server* s, const float* pts, uint32_t n, floa
         return TC_ERR_OVER_BUDGET;
     }
     const double t = s->now();
-    const uint64_t rid = s->next_id++;
+    const uint64_t rid = s->next_id;
+    if (!s->sched.push({rid, n, reserve, t})) {
+        ++s->rejected;
+        return TC_ERR_BUSY;  // back-pressure: max_queued requests already wait
+    }
+    ++s->next_id;
     s->inputs[rid] = {pts, min_coord, range};
-    s->sched.push({rid, n, reserve, t});
     if (s->first_submit < 0) s->first_submit = t;
     ++s->submitted;
     ++s->gen;
     s->cv_work.notify_one();
     *id = rid;
     return TC_OK;
+}
+
+extern "C" int tc_server_cancel(tc_server* s, uint64_t id) {
+    if (!s) return TC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->sched.remove(id)) {
+        s->inputs.erase(id);
+        tc_server_result res{};
+        res.id = id;
+        res.status = TC_ERR_CANCELLED;
+        res.t_done = s->now();
+        s->finished.push_back(res);
+        ++s->cancelled;
+        ++s->completed;
+        s->cv_done.notify_all();
+        return TC_OK;
+    }
+    for (const Slot& sl : s->slots)
+        for (const Live& l : sl.reqs)
+            if (l.r.id == id) return TC_ERR_BUSY;  // being built
+    return TC_ERR_NOT_FOUND;
 }
 
 extern "C" int tc_server_poll(tc_server* s, tc_server_result* out, uint32_t max, uint32_t* n_out, int wait_ms) {
@@ -426,6 +453,8 @@ extern "C" int tc_server_stats_get(tc_server* s, tc_server_stats* st) {
     st->reserved_bytes = s->sched.reserved();
     st->peak_reserved_bytes = s->sched.peak();
     st->result_budget_bytes = s->sched.limits().budget;
+    st->cancelled = s->cancelled;
+    st->queued = s->sched.queued();
     return TC_OK;
 }
 
@@ -433,7 +462,7 @@ extern "C" int tc_server_stats_reset(tc_server* s) {
     if (!s) return TC_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
     s->submitted -= s->completed;
-    s->completed = s->rejected = s->batches = s->clouds_launched = s->points_done = 0;
+    s->completed = s->rejected = s->cancelled = s->batches = s->clouds_launched = s->points_done = 0;
     s->lat_ms.clear();
     s->first_submit = -1.0;
     s->last_done = 0.0;

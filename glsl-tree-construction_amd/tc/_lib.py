@@ -18,11 +18,11 @@ TC_OK = 0
 STATUS_NAMES = {
     0: "TC_OK", -1: "TC_ERR_INVALID", -2: "TC_ERR_NOMEM", -3: "TC_ERR_TOO_LARGE", -4: "TC_ERR_HIP",
     -5: "TC_ERR_OVER_BUDGET", -6: "TC_ERR_CAPACITY", -7: "TC_ERR_BUSY", -8: "TC_ERR_NOT_FOUND",
-    -9: "TC_ERR_NO_DEVICE", -10: "TC_ERR_BAD_TREE", -11: "TC_ERR_IO",
+    -9: "TC_ERR_NO_DEVICE", -10: "TC_ERR_BAD_TREE", -11: "TC_ERR_IO", -12: "TC_ERR_CANCELLED",
 }
 TC_ERR_INVALID, TC_ERR_NOMEM, TC_ERR_TOO_LARGE, TC_ERR_HIP = -1, -2, -3, -4
 TC_ERR_OVER_BUDGET, TC_ERR_CAPACITY, TC_ERR_BUSY, TC_ERR_NOT_FOUND, TC_ERR_NO_DEVICE = -5, -6, -7, -8, -9
-TC_ERR_BAD_TREE, TC_ERR_IO = -10, -11
+TC_ERR_BAD_TREE, TC_ERR_IO, TC_ERR_CANCELLED = -10, -11, -12
 
 OCT_NODE_DTYPE = np.dtype(
     [
@@ -149,6 +149,7 @@ class TcServerConfig(C.Structure):
         ("max_batch_points", C.c_uint32),
         ("code_bits", C.c_int),
         ("link_leaves", C.c_int),
+        ("max_queued", C.c_uint32),
     ]
 
 
@@ -183,6 +184,8 @@ class TcServerStats(C.Structure):
         ("reserved_bytes", C.c_uint64),
         ("peak_reserved_bytes", C.c_uint64),
         ("result_budget_bytes", C.c_uint64),
+        ("cancelled", C.c_uint64),
+        ("queued", C.c_uint64),
     ]
 
 
@@ -196,6 +199,9 @@ class TcSimConfig(C.Structure):
         ("launch_s", C.c_double),
         ("per_point_s", C.c_double),
         ("hold_s", C.c_double),
+        ("max_queued", C.c_uint32),
+        ("alloc_fail_after", C.c_uint32),
+        ("alloc_fail_count", C.c_uint32),
     ]
 
 
@@ -306,6 +312,8 @@ SIGNATURES = {
     "tc_server_run_trace": (I, [P, U32, P, P, P, F, F, P]),
     "tc_plain_run_trace": (I, [P, U32, P, P, P, F, F, I, P]),
     "tc_server_simulate": (I, [C.POINTER(TcSimConfig), U32, P, P, P, P, P, C.POINTER(U64)]),
+    "tc_server_simulate_ex": (I, [C.POINTER(TcSimConfig), U32, P, P, P, P, P, P, P, C.POINTER(U64)]),
+    "tc_server_cancel": (I, [P, U64]),
     "tc_gen_uniform": (None, [P, U64, U64, F]),
     "tc_gen_gaussian_mixture": (None, [P, U64, U64, U32, F, F]),
 }

@@ -39,7 +39,7 @@ def result_bytes(n: int, code_bits: int = 30) -> int:
 class Server:
     def __init__(self, ctx: Context, mem_budget_bytes: int = 64 << 30, n_streams: int = 2,
                  max_batch_clouds: int = 256, max_batch_points: int = 1 << 24, code_bits: int = 30,
-                 link_leaves: bool = True):
+                 link_leaves: bool = True, max_queued: int = 1 << 16):
         cfg = TcServerConfig()
         lib().tc_server_config_default(C.byref(cfg))
         cfg.mem_budget_bytes = mem_budget_bytes
@@ -48,6 +48,7 @@ class Server:
         cfg.max_batch_points = max_batch_points
         cfg.code_bits = code_bits
         cfg.link_leaves = int(bool(link_leaves))
+        cfg.max_queued = max_queued
         self.ctx = ctx
         self.code_bits = code_bits
         self._h = C.c_void_p()
@@ -82,6 +83,12 @@ class Server:
                             tree={f: getattr(r.tree, f) for f in RESULT_FIELDS},
                             t_submit=r.t_submit, t_start=r.t_start, t_done=r.t_done))
         return out
+
+    def cancel(self, rid: int):
+        """Drop a request that has not started (TcError TC_ERR_BUSY if it is
+        being built, TC_ERR_NOT_FOUND if unknown or finished); it then comes
+        back from poll with status TC_ERR_CANCELLED."""
+        check(lib().tc_server_cancel(self._h, rid), "tc_server_cancel")
 
     def release(self, rid: int):
         check(lib().tc_server_release(self._h, rid), "tc_server_release")
@@ -140,16 +147,27 @@ def plain_run_trace(ctx: Context, t_arrive, pts_list, n_list, min_coord=0.0, ran
 
 
 def simulate(t_arrive, n, n_streams=2, max_batch_clouds=256, max_batch_points=1 << 24,
-             result_budget_bytes=64 << 30, code_bits=30, launch_s=50e-6, per_point_s=1e-10, hold_s=0.0) -> dict:
+             result_budget_bytes=64 << 30, code_bits=30, launch_s=50e-6, per_point_s=1e-10, hold_s=0.0,
+             max_queued=0, alloc_fail_after=0, alloc_fail_count=0, t_cancel=None, check_status=True) -> dict:
+    """Simulated run of the server's policy (tc_server_simulate_ex).  Returns
+    per-request t_start / t_done / batch_of / status and the peak reservation;
+    with check_status=False a run that ends with requests stuck (status
+    TC_ERR_NOMEM) returns instead of raising."""
     cfg = TcSimConfig(n_streams, max_batch_clouds, max_batch_points, result_budget_bytes, code_bits, launch_s,
-                      per_point_s, hold_s)
+                      per_point_s, hold_s, max_queued, alloc_fail_after, alloc_fail_count)
     t = np.ascontiguousarray(t_arrive, np.float64)
     nn = np.ascontiguousarray(n, np.uint32)
     k = len(t)
+    tc_ = None if t_cancel is None else np.ascontiguousarray(t_cancel, np.float64)
     t_start = np.zeros(k, np.float64)
     t_done = np.zeros(k, np.float64)
     batch_of = np.zeros(k, np.uint32)
+    status = np.zeros(k, np.int32)
     peak = C.c_uint64()
-    check(lib().tc_server_simulate(C.byref(cfg), k, t.ctypes.data, nn.ctypes.data, t_start.ctypes.data,
-                                   t_done.ctypes.data, batch_of.ctypes.data, C.byref(peak)), "tc_server_simulate")
-    return dict(t_start=t_start, t_done=t_done, batch_of=batch_of, peak_reserved=peak.value)
+    st = lib().tc_server_simulate_ex(C.byref(cfg), k, t.ctypes.data, nn.ctypes.data,
+                                     None if tc_ is None else tc_.ctypes.data, t_start.ctypes.data,
+                                     t_done.ctypes.data, batch_of.ctypes.data, status.ctypes.data, C.byref(peak))
+    if check_status or st != -2:
+        check(st, "tc_server_simulate_ex")
+    return dict(t_start=t_start, t_done=t_done, batch_of=batch_of, status=status, peak_reserved=peak.value,
+                result=st)

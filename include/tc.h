@@ -50,7 +50,8 @@ enum tc_status {
     TC_ERR_NOT_FOUND = -8,   /* unknown request id */
     TC_ERR_NO_DEVICE = -9,   /* no usable GPU */
     TC_ERR_BAD_TREE = -10,   /* tc_tree_validate*: an invariant does not hold */
-    TC_ERR_IO = -11          /* file could not be written */
+    TC_ERR_IO = -11,         /* file could not be written */
+    TC_ERR_CANCELLED = -12   /* served request cancelled before it started */
 };
 
 #define TC_MAX_SORT_N (1u << 30)
@@ -500,6 +501,8 @@ typedef struct {
     uint32_t max_batch_points;  /* per batch, also the largest request (default 2^24) */
     int code_bits;              /* 30 or 63 (default 30) */
     int link_leaves;            /* default 1 */
+    uint32_t max_queued;        /* requests waiting for launch before submit returns TC_ERR_BUSY
+                                   (default 65536; 0: unbounded) */
 } tc_server_config;
 
 typedef struct {
@@ -517,6 +520,8 @@ typedef struct {
     double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;
     double first_submit_s, last_done_s;  /* throughput = points_completed / (last_done - first_submit) */
     uint64_t reserved_bytes, peak_reserved_bytes, result_budget_bytes;
+    uint64_t cancelled;  /* requests cancelled before they started (counted as completed) */
+    uint64_t queued;     /* requests waiting for launch now */
 } tc_server_stats;
 
 /* Fills cfg with the defaults. */
@@ -527,9 +532,15 @@ int tc_server_create(tc_ctx* ctx, const tc_server_config* cfg, tc_server** out);
 int tc_server_destroy(tc_server* srv);
 /* Queue a build of n points (device, 16-byte aligned xyzw; must stay valid
  * until the result is polled).  TC_ERR_CAPACITY: n above max_batch_points;
- * TC_ERR_OVER_BUDGET: its result could never fit the budget. */
+ * TC_ERR_OVER_BUDGET: its result could never fit the budget; TC_ERR_BUSY:
+ * max_queued requests are already waiting (back-pressure: retry later). */
 int tc_server_submit(tc_server* srv, const float* pts_xyzw, uint32_t n, float min_coord, float range,
                      uint64_t* id);
+/* Cancel a request that has not started: it is dropped and comes back from
+ * tc_server_poll with status TC_ERR_CANCELLED and no arrays (its points need
+ * not stay valid after this returns TC_OK).  TC_ERR_BUSY: it is being built
+ * (poll for its result); TC_ERR_NOT_FOUND: unknown or already finished. */
+int tc_server_cancel(tc_server* srv, uint64_t id);
 /* Move up to max finished results to out (wait up to wait_ms for the first). */
 int tc_server_poll(tc_server* srv, tc_server_result* out, uint32_t max, uint32_t* n_out, int wait_ms);
 /* Return a polled result's device memory to the budget. */
@@ -560,9 +571,26 @@ typedef struct {
     uint64_t result_budget_bytes;
     int code_bits;
     double launch_s, per_point_s, hold_s;
+    /* admission back-pressure, as tc_server_config.max_queued (0: unbounded) */
+    uint32_t max_queued;
+    /* failure injection into the result allocator (an arena of
+     * result_budget_bytes, as in the server): after alloc_fail_after successful
+     * allocations the next alloc_fail_count attempts fail (0xFFFFFFFF: all of
+     * them); a failed placement ends the batch and the request waits. */
+    uint32_t alloc_fail_after, alloc_fail_count;
 } tc_sim_config;
+/* Outputs per request: t_start / t_done (-1 if it never ran), batch_of
+ * (0xFFFFFFFF if none); status (optional): TC_OK, TC_ERR_BUSY (rejected at
+ * admission), TC_ERR_CAPACITY / TC_ERR_OVER_BUDGET (could never be served),
+ * TC_ERR_CANCELLED (t_cancel reached while it waited) or TC_ERR_NOMEM (still
+ * waiting when no event could free memory: the call then returns TC_ERR_NOMEM).
+ * t_cancel (optional): per request, the time its client cancels it (+inf:
+ * never). */
 int tc_server_simulate(const tc_sim_config* cfg, uint32_t n_req, const double* t_arrive, const uint32_t* n,
                        double* t_start, double* t_done, uint32_t* batch_of, uint64_t* peak_reserved);
+int tc_server_simulate_ex(const tc_sim_config* cfg, uint32_t n_req, const double* t_arrive, const uint32_t* n,
+                          const double* t_cancel, double* t_start, double* t_done, uint32_t* batch_of,
+                          int32_t* status, uint64_t* peak_reserved);
 
 /* ---- Host-array mirrors of the reference's per-stage functions (blocking).
  * Argument order and meaning follow the Rust functions cited; sizes are

@@ -261,3 +261,53 @@ def test_served_workload_sizes_poisson_match_oracle(gpu_ctx, oracle, tclib):
         assert srv.stats()["reserved_bytes"] == 0
     finally:
         srv.close()
+
+
+def test_backpressure_and_cancellation(gpu_ctx, oracle, tclib):
+    """The live server's bounded admission queue and cancellation: with one
+    stream busy on a large build, submissions beyond max_queued waiting
+    requests get TC_ERR_BUSY; a waiting request can be cancelled (it comes back
+    with TC_ERR_CANCELLED and no arrays), and the others still equal the oracle."""
+    import tc
+
+    big = dev(tclib.gen_uniform(1 << 22, 901))
+    small = [tclib.gen_uniform(5000 + 100 * i, 910 + i) for i in range(10)]
+    d_small = [dev(c) for c in small]
+    srv = tc.Server(gpu_ctx, mem_budget_bytes=16 << 30, n_streams=1, max_batch_clouds=1,
+                    max_batch_points=1 << 22, max_queued=4)
+    try:
+        first = srv.submit(big)
+        ids, busy = [], 0
+        for i, d in enumerate(d_small):
+            try:
+                ids.append((i, srv.submit(d)))
+            except tc.TcError as e:
+                assert e.status == tc._lib.TC_ERR_BUSY
+                busy += 1
+        assert len(ids) <= 5 and busy >= 5  # at most the queue bound (+1 if the big build had not started)
+        # Cancel the last accepted request: still waiting behind the big build.
+        i_c, id_c = ids[-1]
+        cancelled = True
+        try:
+            srv.cancel(id_c)
+        except tc.TcError as e:  # it could only have started if the big build were already done
+            assert e.status in (tc._lib.TC_ERR_BUSY, tc._lib.TC_ERR_NOT_FOUND)
+            cancelled = False
+        with pytest.raises(tc.TcError):
+            srv.cancel(10 ** 9)  # unknown id
+        got = collect(srv, [first] + [r for _, r in ids])
+        st = srv.stats()
+        if cancelled:
+            assert got[id_c]["status"] == tc._lib.TC_ERR_CANCELLED and got[id_c]["n_nodes"] == 0
+            assert st["cancelled"] == 1
+        assert st["rejected"] == busy and st["queued"] == 0
+        for i, rid in ids:
+            if cancelled and rid == id_c:
+                continue
+            assert got[rid]["status"] == 0
+            assert_same_tree(srv.download(got[rid]), oracle_tree(oracle, small[i]), f"request {i}")
+            srv.release(rid)
+        srv.release(first)
+        assert srv.stats()["reserved_bytes"] == 0
+    finally:
+        srv.close()

@@ -113,3 +113,63 @@ def test_result_bytes_cover_the_layout(tc):
     assert b == want
     want63 = up(8 * n) + 3 * up(n) + 4 * up(4 * n) + up(64 * tc.octree_node_bound(n, 63))
     assert tc.result_bytes(n, 63) == want63
+
+
+def test_queue_saturation_rejects_with_busy(tc):
+    """A bounded admission queue: arrivals beyond max_queued waiting requests are
+    rejected (TC_ERR_BUSY) instead of queueing without bound; the others run in
+    order, and with room to spare nothing is rejected."""
+    t = np.arange(200) * 1e-7  # a burst far faster than the modelled device
+    n = np.full(200, 1 << 18, np.uint32)
+    r = tc.simulate(t, n, n_streams=1, max_batch_clouds=4, launch_s=1e-4, per_point_s=1e-9, max_queued=16)
+    busy = r["status"] == tc._lib.TC_ERR_BUSY
+    ok = r["status"] == 0
+    assert busy.sum() > 0 and ok.sum() + busy.sum() == 200
+    assert np.all(r["t_start"][busy] < 0) and np.all(r["t_done"][ok] > 0)
+    assert np.all(np.diff(r["t_start"][ok]) >= 0)
+    # Never more than 16 waiting: every accepted request was admitted while
+    # fewer than 16 earlier accepted ones had not yet started.
+    started = r["t_start"][ok]
+    for i, ta in enumerate(t[ok]):
+        waiting = np.sum((t[ok][:i] <= ta) & (started[:i] > ta))
+        assert waiting < 16
+    free = tc.simulate(t, n, n_streams=1, max_batch_clouds=4, launch_s=1e-4, per_point_s=1e-9, max_queued=1000)
+    assert np.all(free["status"] == 0)
+
+
+def test_cancellation_of_waiting_requests(tc):
+    """Requests cancelled while they wait never start; the rest are served as
+    without them, and a cancel after the start has no effect."""
+    t = np.arange(40) * 1e-6
+    n = np.full(40, 1 << 20, np.uint32)
+    t_cancel = np.full(40, np.inf)
+    t_cancel[10:20] = 1e-4  # still queued behind the first batches then
+    t_cancel[0] = 1.0  # long after it started
+    r = tc.simulate(t, n, n_streams=1, max_batch_clouds=2, launch_s=1e-4, per_point_s=1e-9, t_cancel=t_cancel)
+    cancelled = r["status"] == tc._lib.TC_ERR_CANCELLED
+    assert cancelled[10:20].all() and cancelled.sum() == 10
+    assert np.all(r["t_start"][cancelled] < 0)
+    kept = ~cancelled
+    assert np.all(r["t_done"][kept] > 0) and np.all(np.diff(r["t_start"][kept]) >= 0)
+    base = tc.simulate(t, n, n_streams=1, max_batch_clouds=2, launch_s=1e-4, per_point_s=1e-9)
+    assert r["t_done"][kept][-1] < base["t_done"][-1]  # the cancelled work is not done
+
+
+def test_allocator_failing_after_k_allocations(tc):
+    """Injected result-allocator failures: after k placements the next ones
+    fail; a failed placement ends the batch and the request waits (in order),
+    so a finite run of failures only delays requests, and a permanent one
+    leaves the remaining requests unserved (TC_ERR_NOMEM) instead of hanging."""
+    t = np.arange(30) * 1e-6
+    n = np.full(30, 1 << 16, np.uint32)
+    kw = dict(n_streams=2, max_batch_clouds=4, launch_s=1e-4, per_point_s=1e-9)
+    base = tc.simulate(t, n, **kw)
+    delayed = tc.simulate(t, n, alloc_fail_after=5, alloc_fail_count=7, **kw)
+    assert np.all(delayed["status"] == 0) and np.all(delayed["t_done"] > 0)
+    assert np.all(np.diff(delayed["t_start"]) >= 0)
+    assert np.array_equal(delayed["t_start"][:5], base["t_start"][:5])
+    assert np.all(delayed["t_start"] >= base["t_start"] - 1e-15)
+    stuck = tc.simulate(t, n, alloc_fail_after=5, alloc_fail_count=0xFFFFFFFF, check_status=False, **kw)
+    assert stuck["result"] == tc._lib.TC_ERR_NOMEM
+    assert np.sum(stuck["status"] == 0) == 5 and np.sum(stuck["status"] == tc._lib.TC_ERR_NOMEM) == 25
+    assert np.all(stuck["t_start"][5:] < 0)
