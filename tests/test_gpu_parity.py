@@ -509,3 +509,53 @@ def test_scan_modes_at_tile_edges(gpu_ctx, oracle, n):
             want_ec = oracle.edge_count(t["prefix_n"], t["parent"])
             assert np.array_equal(host(ec, np.uint32), want_ec)
             assert np.array_equal(host(offs, np.uint32), oracle.exclusive_scan(want_ec))
+
+
+@pytest.mark.parametrize("block,kpt", [(1024, 8), (1024, 16)])
+@pytest.mark.parametrize("n", [1, 2, 3, 8191, 8192, 8193, 16384, 16385, 49153, 1 << 20])
+def test_wide_digit_plans_at_tile_edges(gpu_ctx, oracle, tclib, block, kpt, n):
+    """The wide digit plans (three passes of <= 11 bits for u32 keys, six for
+    u64 keys) at sizes around their tile edges, against the oracle: u32 and u64
+    key sorts with and without payload (odd pass counts copy back), and the
+    30- and 63-bit Morton sorts (3 x 10 and 3 x 11 + 3 x 10 bits) with the
+    point index, and a whole tree build under the wide plan."""
+    import tc
+
+    r = np.random.default_rng(n * 7 + kpt)
+    keys = r.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    keys[::5] = keys[0]
+    vals = np.arange(n, dtype=np.uint32)
+    k64 = r.integers(0, 1 << 64, size=n, dtype=np.uint64)
+    prev = gpu_ctx.sort_plan()
+    gpu_ctx.set_sort_plan(block, kpt, 11)
+    try:
+        sk, sv = oracle.radix_sort(keys, vals)
+        k, v = dev(keys), dev(vals)
+        gpu_ctx.sort(k, empty(n, np.uint32), v, empty(n, np.uint32))
+        assert np.array_equal(host(k, np.uint32, n), sk) and np.array_equal(host(v, np.uint32, n), sv)
+        k = dev(k64)
+        gpu_ctx.sort(k, empty(n, np.uint64))
+        assert np.array_equal(host(k, np.uint64, n), np.sort(k64))
+        k, v = dev(k64), dev(vals)
+        gpu_ctx.sort(k, empty(n, np.uint64), v, empty(n, np.uint32))
+        sk, sv = oracle.radix_sort(k64, vals)
+        assert np.array_equal(host(k, np.uint64, n), sk) and np.array_equal(host(v, np.uint32, n), sv)
+        pts = tclib.gen_gaussian_mixture(n, n + 3, 8, 3.0)
+        d = dev(pts)
+        for cb, kdt in ((30, np.uint32), (63, np.uint64)):
+            codes = oracle.compute_morton(pts, 0.0, 1024.0) if cb == 30 else oracle.compute_morton63(pts, 0.0, 1024.0)
+            want_k, want_v = oracle.radix_sort(codes, vals)
+            mk, mv = empty(n, kdt), empty(n, np.uint32)
+            gpu_ctx.morton_sort_pairs(d, mk, mv, empty(n, kdt), empty(n, np.uint32), 0.0, 1024.0)
+            assert np.array_equal(host(mk, kdt, n), want_k), cb
+            assert np.array_equal(host(mv, np.uint32, n), want_v), cb
+        if n >= 2:
+            tree = tc.Tree(gpu_ctx, n, 30)
+            tree.build(d, n, 0.0, 1024.0)
+            got = tree.download()
+            tree.free()
+            want = oracle_all(oracle, pts, 0.0, 1024.0, 30)
+            assert_same(got, want, ("sorted", "unique", "prefix_n", "left_child", "parent", "edge_count", "offsets"))
+            assert np.array_equal(got["nodes"].view(np.uint8).reshape(-1), want["nodes"])
+    finally:
+        gpu_ctx.set_sort_plan(*prev)
