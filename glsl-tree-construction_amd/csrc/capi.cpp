@@ -245,7 +245,7 @@ extern "C" int tc_create(tc_ctx** out, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cus = prop.multiProcessorCount;
-    const size_t status_bytes = ((size_t)(1u << 18) + 1) * sizeof(uint64_t);
+    const size_t status_bytes = tc_scan_status_bytes();
     const size_t acc_bytes = tc_hist_acc_bytes();
     if (hipMalloc(&ctx->small, SMALL_WORDS * 4) != hipSuccess ||
         hipMalloc(&ctx->scan_status, status_bytes) != hipSuccess ||

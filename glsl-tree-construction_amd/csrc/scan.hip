@@ -8,19 +8,33 @@
 // shader/edge_count.comp:30-41.  Published algorithm: Merrill & Garland,
 // "Single-pass Parallel Prefix Scan with Decoupled Look-back" (2016).
 //
-// gfx950 design: one 8192-element tile per workgroup, tile = workgroup id (a
-// claim counter would serialise at about 11 ns per claim on MI355X).  A tile is
-// loaded with 16 B vector loads (each wave instruction reads 1 KiB contiguous)
-// into LDS and read back so each thread owns 16 consecutive elements; a block
-// scan gives tile-local offsets; wave 0 publishes the tile aggregate and looks
-// back over 64 predecessors per round trip.  The wait is bounded: a workgroup
-// that has polled spin_limit times without an answer reduces the elements
-// before its tile itself (exact, slower), so progress never depends on the
-// order in which workgroups start.  Results go back through LDS so stores are
-// contiguous too (unique compacts the tile's keys in LDS first).
-// Tile status words are u64 [value:32 | flag:2 | epoch:30]; a one-lane kernel
-// advances the device-side epoch before each scan, so status words never need
-// clearing (also under graph replay).
+// gfx950 design.  Tile = workgroup id (a claim counter would serialise at
+// about 11 ns per claim on MI355X).  Tile status words are u64 [value:32 |
+// flag:2 | epoch:30], one per 64-byte line: adjacent tiles run on different
+// XCDs, and sharing lines between them made each look-back poll slower (2^26
+// sorted codes, 16K-key tiles: 92.8 -> 77.9 us).  A one-lane kernel advances
+// the device-side epoch before each scan, so status words never need clearing
+// (also under graph replay).  Wave 0 looks back over 64 predecessors per poll
+// (measured: reading 2-4 words per lane, or spreading the poll over 2-8 waves,
+// halves the polls but makes each slower, a net loss).  The wait is bounded: a
+// workgroup that has polled spin_limit times without an answer reduces the
+// elements before its tile itself (exact, slower), so progress never depends
+// on the order in which workgroups start.
+//
+// Unique (k_unique_tiles): 512 threads x 8 rows of 16-byte vectors (16384 u32
+// or 8192 u64 keys per tile) loaded straight into registers; the key before
+// each wave's run of a row comes from a scalar load, the others from the lane
+// below.  Run-start flags are counted per row in 16-bit fields of two u64s,
+// so one block scan gives every row's offsets.  The tile publishes its count,
+// compacts its first four rows into LDS, then looks back; the stores after it
+// are contiguous.  (Measured on 2^26 sorted 30-bit codes of the Gaussian
+// mixture: 132 us for the LDS-staged 8192-key tiles this replaces; 61.8 us with
+// exact prefixes given and no look-back at all.)
+//
+// Scans (k_scan_tiles): one 8192-element tile per workgroup, loaded with 16 B
+// vector loads into LDS and read back so each thread owns 16 consecutive
+// elements; a block scan gives tile-local offsets.  Results go back through
+// LDS so stores are contiguous too.
 //
 // Segmented modes run one scan over many concatenated clouds: an element that
 // starts a cloud always counts as a new unique key, and the exclusive prefix at
@@ -33,6 +47,8 @@ namespace {
 constexpr int SCAN_BLOCK = 512;
 constexpr int SCAN_IPT = 16;
 constexpr uint32_t SCAN_TILE = SCAN_BLOCK * SCAN_IPT;
+// u64 words between two tiles' status words (64 bytes).
+constexpr uint32_t STATUS_STRIDE = 8;
 
 // Scan control words (in ctx->small).
 struct ScanCtl {
@@ -92,7 +108,7 @@ TC_DEV uint32_t wave_lookback(const uint64_t* status, uint32_t tile, uint32_t ep
         uint64_t incl_mask, upto;
         while (true) {
             if (idx >= 0) {
-                const uint64_t w = ld_agent64(&status[idx]);
+                const uint64_t w = ld_agent64(&status[idx * STATUS_STRIDE]);
                 const uint32_t hi = (uint32_t)(w >> 32);
                 f = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
                 v = (uint32_t)w;
@@ -156,20 +172,31 @@ TC_DEV uint32_t elem_value(const ScanArgs& a, uint32_t n, uint32_t i, uint32_t n
     return i == 0 ? 1u : (uint32_t)(pn[i] / 3 - pn[p] / 3);
 }
 
-// Coalesced tile load into LDS (tile-relative positions).  STREAM: the input
-// is read only here (unique's sorted keys), so full tiles load it
-// non-temporally (ld_stream); the scans' inputs are read again later.
-template <typename T, bool STREAM>
-TC_DEV void load_tile(const T* __restrict__ in, uint32_t base, uint32_t cnt, T* s, bool aligned) {
-    constexpr int VEC = 16 / sizeof(T);
+// The look-back's fall-back: the sum of the scan values of [0, base), by the
+// whole block (each thread sums a contiguous chunk).  Every thread calls it.
+template <int MODE>
+TC_DEV uint32_t reduce_before(const ScanArgs& a, uint32_t n, uint32_t nseg, uint32_t base, uint32_t* s_tmp) {
+    const uint32_t chunk = ceil_div_u32(base, SCAN_BLOCK);
+    const uint32_t lo = threadIdx.x * chunk, hi = lo + chunk < base ? lo + chunk : base;
+    uint32_t sum = 0, seg = 0, seg_start = 0, seg_next = 0xFFFFFFFFu;
+    if (is_seg(MODE) && nseg > 0 && lo < hi) {
+        seg = find_seg(a.seg_off, nseg, lo);
+        seg_start = a.seg_off[seg];
+        seg_next = a.seg_off[seg + 1];
+    }
+    for (uint32_t i = lo; i < hi; ++i) sum += elem_value<MODE>(a, n, i, nseg, seg, seg_start, seg_next);
+    uint32_t total;
+    block_exclusive_scan<SCAN_BLOCK>(sum, s_tmp, &total);
+    return total;
+}
+
+// Coalesced tile load of u32 scan values into LDS (tile-relative positions).
+TC_DEV void load_tile(const uint32_t* __restrict__ in, uint32_t base, uint32_t cnt, uint32_t* s, bool aligned) {
     if (aligned && cnt == SCAN_TILE) {
-        using V = typename std::conditional<sizeof(T) == 4, uint4, ulonglong2>::type;
-        const V* vin = reinterpret_cast<const V*>(in + base);
-        V* vs = reinterpret_cast<V*>(s);
+        const uint4* vin = reinterpret_cast<const uint4*>(in + base);
+        uint4* vs = reinterpret_cast<uint4*>(s);
 #pragma unroll
-        for (int r = 0; r < (int)(SCAN_TILE / VEC / SCAN_BLOCK); ++r)
-            vs[r * SCAN_BLOCK + threadIdx.x] =
-                STREAM ? ld_stream(&vin[r * SCAN_BLOCK + threadIdx.x]) : vin[r * SCAN_BLOCK + threadIdx.x];
+        for (int r = 0; r < (int)(SCAN_TILE / 4 / SCAN_BLOCK); ++r) vs[r * SCAN_BLOCK + threadIdx.x] = vin[r * SCAN_BLOCK + threadIdx.x];
     } else {
         for (uint32_t k = threadIdx.x; k < cnt; k += SCAN_BLOCK) s[k] = in[base + k];
     }
@@ -180,14 +207,15 @@ __global__ void k_scan_prep(ScanCtl* ctl) {
     ctl->seq = (ctl->seq + 1u) & 0x3FFFFFFFu;
 }
 
+// Exclusive scans (plain, segmented, and the fused edge count).
 template <int MODE>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
-    using T = ItemT<MODE>;
+    static_assert(!is_unique(MODE), "unique modes run k_unique_tiles");
     __shared__ uint32_t s_tmp[SCAN_BLOCK / 64 + 2];
     __shared__ uint32_t s_excl;
     __shared__ uint32_t s_fallback;
     __shared__ uint32_t s_seg0;  // segmented modes: the segment of the tile's first element
-    __shared__ __attribute__((aligned(16))) T s_items[SCAN_TILE];
+    __shared__ __attribute__((aligned(16))) uint32_t s_items[SCAN_TILE];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6;
@@ -201,7 +229,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         if (n > a.n_cap) n = a.n_cap;
     } else if (a.d_n_keys) {
         const uint32_t u = *a.d_n_keys;
-        const uint32_t m = (MODE == MODE_SCAN || MODE == MODE_ECSCAN) ? (u > 0 ? u - 1 : 0) : u;
+        const uint32_t m = u > 0 ? u - 1 : 0;  // the scans run over the n_keys - 1 radix nodes
         n = m < a.n_cap ? m : a.n_cap;
     }
     const uint32_t num_tiles = n ? ceil_div_u32(n, SCAN_TILE) : 1u;
@@ -215,15 +243,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     const uint32_t cnt = n > base ? (n - base < SCAN_TILE ? n - base : SCAN_TILE) : 0u;
     const uint32_t r0 = tid * SCAN_IPT;  // tile-relative index of this thread's first element
     const uint32_t i0 = base + r0;
-    // Per-element values; in the unique modes they are 0/1 run-start flags,
-    // kept as bits of one register instead (V(j) reads either form).
-    static_assert(SCAN_IPT <= 32, "flags of one thread's elements fit one word");
-    uint32_t val[is_unique(MODE) ? 1 : SCAN_IPT];
-    uint32_t keep = 0;
-    auto V = [&](int j) -> uint32_t {
-        if constexpr (is_unique(MODE)) return (keep >> j) & 1u;
-        else return val[j];
-    };
+    uint32_t val[SCAN_IPT];
+    auto V = [&](int j) -> uint32_t { return val[j]; };
     uint32_t start_mask = 0;  // bit j: element i0 + j starts a segment
     uint32_t seg_first = 0;
 
@@ -235,7 +256,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         if (lane_id() == 0) s_seg0 = c;
     }
     if (MODE != MODE_ECSCAN) {  // (segmented modes always take this sync)
-        load_tile<T, is_unique(MODE)>(static_cast<const T*>(a.in0), base, cnt, s_items, aligned);
+        load_tile(static_cast<const uint32_t*>(a.in0), base, cnt, s_items, aligned);
         __syncthreads();
     }
     if (is_seg(MODE) && nseg > 0 && i0 < n) {
@@ -257,18 +278,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
 
     if (is_plain_values(MODE)) {
 #pragma unroll
-        for (int j = 0; j < SCAN_IPT; ++j) val[j] = (r0 + j < cnt) ? (uint32_t)s_items[r0 + j] : 0u;
-    } else if (is_unique(MODE)) {
-        T prev = (T)0;
-        if (r0 < cnt) prev = r0 > 0 ? s_items[r0 - 1] : (base > 0 ? static_cast<const T*>(a.in0)[base - 1] : (T)0);
-#pragma unroll
-        for (int j = 0; j < SCAN_IPT; ++j) {
-            const uint32_t r = r0 + j;
-            const T k = r < cnt ? s_items[r] : (T)0;
-            const bool starts = (base + r == 0) || ((start_mask >> j) & 1u);
-            keep |= ((r < cnt && (starts || k != prev)) ? 1u : 0u) << j;
-            prev = k;
-        }
+        for (int j = 0; j < SCAN_IPT; ++j) val[j] = (r0 + j < cnt) ? s_items[r0 + j] : 0u;
     } else {  // MODE_ECSCAN: edge count of radix node i, root entry 1
         uint32_t z0 = 0, z1 = 0, z2 = 0;
 #pragma unroll
@@ -291,7 +301,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
                 s_excl = 0;
             }
         } else {
-            if (lane == 0) st_agent64(&a.status[tile], st_word(1u, epoch, block_total));
+            if (lane == 0) st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(1u, epoch, block_total));
             if (!s_fallback) {
                 bool gave_up = false;
                 const uint32_t ex = wave_lookback(a.status, tile, epoch, a.spin_limit, &gave_up);
@@ -304,25 +314,14 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     }
     __syncthreads();
     if (s_fallback) {
-        // Reduce [0, base) directly: each thread takes a contiguous chunk.
-        const uint32_t chunk = ceil_div_u32(base, SCAN_BLOCK);
-        const uint32_t lo = tid * chunk, hi = lo + chunk < base ? lo + chunk : base;
-        uint32_t sum = 0, seg = 0, seg_start = 0, seg_next = 0xFFFFFFFFu;
-        if (is_seg(MODE) && nseg > 0 && lo < hi) {
-            seg = find_seg(a.seg_off, nseg, lo);
-            seg_start = a.seg_off[seg];
-            seg_next = a.seg_off[seg + 1];
-        }
-        for (uint32_t i = lo; i < hi; ++i) sum += elem_value<MODE>(a, n, i, nseg, seg, seg_start, seg_next);
-        uint32_t before;
-        block_exclusive_scan<SCAN_BLOCK>(sum, s_tmp, &before);
+        const uint32_t before = reduce_before<MODE>(a, n, nseg, base, s_tmp);
         if (tid == 0) {
             s_excl = before;
-            st_agent64(&a.status[tile], st_word(2u, epoch, before + block_total));
+            st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, before + block_total));
         }
         __syncthreads();
     } else if (wave == 0 && tile > 0 && lane_id() == 0) {
-        st_agent64(&a.status[tile], st_word(2u, epoch, s_excl + block_total));
+        st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, s_excl + block_total));
     }
     const uint32_t tile_excl = s_excl;
 
@@ -354,39 +353,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         __syncthreads();
         uint32_t* out = static_cast<uint32_t*>(a.out0);
         for (uint32_t k = tid; k < cnt; k += SCAN_BLOCK) out[base + k] = s_out[k];
-    } else if (is_unique(MODE)) {
-        // Compact the tile's kept keys in LDS, then store them contiguously.
-        // The keys are read back from the LDS copy of the tile only now (it is
-        // unchanged since the load), so they are not held in registers through
-        // the scan and the look-back.  Slots past cnt are read but not kept.
-        T item[SCAN_IPT];
-#pragma unroll
-        for (int j = 0; j < SCAN_IPT; ++j) item[j] = s_items[r0 + j];
-        uint32_t run = texcl;
-        __syncthreads();  // everyone has read its items
-#pragma unroll
-        for (int j = 0; j < SCAN_IPT; ++j) {
-            if (V(j)) s_items[run] = item[j];
-            run += V(j);
-        }
-        __syncthreads();
-        T* out = static_cast<T*>(a.out0);
-        for (uint32_t k = tid; k < block_total; k += SCAN_BLOCK) out[tile_excl + k] = s_items[k];
-        if (a.out1) {
-            // Run starts: the position of each kept key (its first occurrence),
-            // compacted the same way.
-            uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_items);
-            run = texcl;
-            __syncthreads();  // the keys are out of LDS
-#pragma unroll
-            for (int j = 0; j < SCAN_IPT; ++j) {
-                if (V(j)) s_pos[run] = i0 + j;
-                run += V(j);
-            }
-            __syncthreads();
-            uint32_t* first = static_cast<uint32_t*>(a.out1);
-            for (uint32_t k = tid; k < block_total; k += SCAN_BLOCK) first[tile_excl + k] = s_pos[k];
-        }
     } else {
         uint32_t run = tile_excl + texcl;
         uint32_t* ec = static_cast<uint32_t*>(a.out0);
@@ -404,7 +370,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
         const uint32_t total = tile_excl + block_total;
         if (is_plain_values(MODE)) static_cast<uint32_t*>(a.out0)[n] = total;
         if (MODE == MODE_ECSCAN) static_cast<uint32_t*>(a.out1)[n] = total;
-        if (is_unique(MODE) && a.out1) static_cast<uint32_t*>(a.out1)[total] = n;  // end of the last run
         // MODE_SCAN reports [element count, total] (the build's n_brt and
         // octree node count); the other modes their total.
         if (a.count_out) {
@@ -418,16 +383,305 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     }
 }
 
+
+// ------------------------------------------------------------------ unique
+constexpr int UQ_ROWS = 8;
+constexpr int SCAN_WAVES = SCAN_BLOCK / 64;
+
+template <typename T>
+struct UqShape {
+    static constexpr uint32_t VEC = 16 / sizeof(T);       // keys per 16-byte load
+    static constexpr uint32_t ROW = SCAN_BLOCK * VEC;      // keys per row
+    static constexpr uint32_t TILE = ROW * UQ_ROWS;        // 16384 u32 / 8192 u64
+    static constexpr uint32_t HALF = ROW * (UQ_ROWS / 2);  // keys compacted per LDS round
+};
+
+uint32_t unique_tile_keys(int mode) {
+    return (mode == MODE_UNIQUE64 || mode == MODE_SEG_UNIQUE64) ? UqShape<uint64_t>::TILE : UqShape<uint32_t>::TILE;
+}
+
+// Inclusive wave scan of four 16-bit counters packed in a u64 (no field
+// overflows: a row holds at most 2048 keys).
+TC_DEV uint64_t wave_inclusive_scan64(uint64_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t t = __shfl_up(v, off, 64);
+        if (lane >= (uint32_t)off) v += t;
+    }
+    return v;
+}
+
+TC_DEV void unpack16(const uint4& v, uint32_t (&k)[4]) {
+    k[0] = v.x;
+    k[1] = v.y;
+    k[2] = v.z;
+    k[3] = v.w;
+}
+TC_DEV void unpack16(const ulonglong2& v, uint64_t (&k)[2]) {
+    k[0] = v.x;
+    k[1] = v.y;
+}
+
+// Unique compaction of sorted keys (plain and segmented).  Slot q of thread t
+// in row r holds key base + r * ROW + t * VEC + q, so a tile is visited in key
+// order row by row; a run starts at a key that differs from the one before it
+// or that starts a cloud.
+template <int MODE>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_unique_tiles(ScanArgs a) {
+    using T = ItemT<MODE>;
+    using Sh = UqShape<T>;
+    using V = typename std::conditional<sizeof(T) == 4, uint4, ulonglong2>::type;
+    constexpr bool SEG = is_seg(MODE);
+    constexpr uint32_t VEC = Sh::VEC, ROW = Sh::ROW, TILE = Sh::TILE;
+    constexpr int NP = UQ_ROWS / 4;  // u64 counter words per thread
+    constexpr int HALF_ROWS = UQ_ROWS / 2;
+    constexpr uint32_t VMASK = (1u << VEC) - 1u;
+    static_assert(UQ_ROWS * VEC <= 32, "one flag word per thread");
+    static_assert(TILE <= 65536, "tile-local offsets fit 16 bits");
+
+    __shared__ uint64_t s_w[NP * SCAN_WAVES];
+    __shared__ uint32_t s_tmp[SCAN_WAVES + 2];
+    __shared__ uint32_t s_excl;
+    __shared__ uint32_t s_fallback;
+    __shared__ uint32_t s_seg[2];  // segmented: the clouds starting in this tile, [s_seg[0], s_seg[1])
+    __shared__ __attribute__((aligned(16))) T s_keys[Sh::HALF];
+    // Segmented: for each thread's slots of each row, the tile-local output
+    // index of its first kept key (low 16 bits) and its run-start flags (bits
+    // 16 and up); and a bitmap of the tile's cloud starts.
+    __shared__ uint32_t s_chunk[SEG ? UQ_ROWS * SCAN_BLOCK : 1];
+    __shared__ uint32_t s_start[SEG ? TILE / 32 : 1];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const uint32_t lane = lane_id();
+
+    uint32_t n = a.n_cap;
+    uint32_t nseg = 0;
+    if (SEG) {
+        nseg = a.d_n_seg ? *a.d_n_seg : a.n_seg_cap;
+        if (nseg > a.n_seg_cap) nseg = a.n_seg_cap;
+        n = nseg ? a.seg_off[nseg] : 0u;
+        if (n > a.n_cap) n = a.n_cap;
+    } else if (a.d_n_keys) {
+        const uint32_t u = *a.d_n_keys;
+        n = u < a.n_cap ? u : a.n_cap;
+    }
+    const uint32_t num_tiles = n ? ceil_div_u32(n, TILE) : 1u;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= num_tiles) return;  // capacity grids: workgroups past the data leave at once
+    const uint32_t epoch = a.ctl->seq & 0x3FFFFFFFu;
+    const uint32_t base = tile * TILE;
+    const uint32_t cnt = n > base ? (n - base < TILE ? n - base : TILE) : 0u;
+    const T* in = static_cast<const T*>(a.in0);
+
+    if (SEG) {
+        for (uint32_t w = tid; w < TILE / 32; w += SCAN_BLOCK) s_start[w] = 0u;
+        // Waves 0 and 1: the first cloud starting at or after the tile's first
+        // key, and at or after its end.
+        if (wave < 2) {
+            const uint32_t x = wave == 0 ? base : base + cnt;
+            uint32_t c = 0;
+            if (nseg > 0 && cnt > 0 && x > 0) c = seg_find_wave(a.seg_off, nseg, x - 1) + 1u;
+            if (lane == 0) s_seg[wave] = c;
+        }
+    }
+
+    // The tile, straight into registers (16-byte loads of keys read once).
+    T k[UQ_ROWS][VEC];
+    const bool aligned = (reinterpret_cast<uintptr_t>(in) & 15u) == 0;
+    if (aligned && cnt == TILE) {
+        const V* vin = reinterpret_cast<const V*>(in + base);
+#pragma unroll
+        for (int r = 0; r < UQ_ROWS; ++r) unpack16(ld_stream(&vin[r * SCAN_BLOCK + tid]), k[r]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < UQ_ROWS; ++r)
+#pragma unroll
+            for (uint32_t q = 0; q < VEC; ++q) {
+                const uint32_t e = r * ROW + tid * VEC + q;
+                k[r][q] = e < cnt ? in[base + e] : (T)0;
+            }
+    }
+    // The key before each wave's slots of a row (a wave-uniform index).
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(wave);
+    T before_wave[UQ_ROWS];
+#pragma unroll
+    for (int r = 0; r < UQ_ROWS; ++r) {
+        const uint32_t e = base + r * ROW + wv * 64 * VEC;
+        before_wave[r] = (e > 0 && e - 1 < n) ? in[e - 1] : (T)0;
+    }
+    if (SEG) {
+        __syncthreads();  // bitmap cleared, s_seg written
+        const uint32_t c_lo = s_seg[0], c_hi = s_seg[1];
+        for (uint32_t c = c_lo + tid; c < c_hi; c += SCAN_BLOCK) {
+            const uint32_t q = a.seg_off[c] - base;
+            atomicOr(&s_start[q >> 5], 1u << (q & 31u));
+        }
+        __syncthreads();
+    }
+
+    // Run-start flags, bit r * VEC + q.
+    uint32_t keep = 0;
+#pragma unroll
+    for (int r = 0; r < UQ_ROWS; ++r) {
+        const T up = __shfl_up(k[r][VEC - 1], 1, 64);
+        T prev = lane == 0 ? before_wave[r] : up;
+        const uint32_t e0 = r * ROW + tid * VEC;
+        const uint32_t starts = SEG ? (s_start[e0 >> 5] >> (e0 & 31u)) & VMASK : 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < VEC; ++q) {
+            const bool run = (base + e0 + q == 0) || ((starts >> q) & 1u) || k[r][q] != prev;
+            keep |= ((e0 + q < cnt && run) ? 1u : 0u) << (r * VEC + q);
+            prev = k[r][q];
+        }
+    }
+    auto row_flags = [&](int r) -> uint32_t { return (keep >> (r * VEC)) & VMASK; };
+
+    // Per-row counts in 16-bit fields, four rows per u64: one block scan gives
+    // every row's offsets.
+    uint64_t pk[NP], inc[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        pk[p] = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pk[p] |= (uint64_t)__popc(row_flags(4 * p + q)) << (16 * q);
+        inc[p] = wave_inclusive_scan64(pk[p]);
+        if (lane == 63) s_w[p * SCAN_WAVES + wave] = inc[p];
+    }
+    if (tid == 0) s_fallback = (a.spin_limit == 0u && tile > 0) ? 1u : 0u;  // 0: always reduce directly
+    __syncthreads();
+    uint32_t row_base[UQ_ROWS];  // tile-local output index of each row's first kept key
+    uint32_t slot[UQ_ROWS];      // ... and of this thread's first kept key in each row
+    uint32_t block_total = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        uint64_t below = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < SCAN_WAVES; ++w) {
+            const uint64_t x = s_w[p * SCAN_WAVES + w];
+            below += ((uint32_t)w < wave) ? x : 0ull;
+            tot += x;
+        }
+        const uint64_t ex = below + inc[p] - pk[p];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            row_base[4 * p + q] = block_total;
+            slot[4 * p + q] = block_total + ((uint32_t)(ex >> (16 * q)) & 0xFFFFu);
+            block_total += (uint32_t)(tot >> (16 * q)) & 0xFFFFu;
+        }
+    }
+    if (wave == 0 && lane == 0)
+        st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(tile == 0 ? 2u : 1u, epoch, block_total));
+    if (SEG) {
+#pragma unroll
+        for (int r = 0; r < UQ_ROWS; ++r) s_chunk[r * SCAN_BLOCK + tid] = slot[r] | (row_flags(r) << 16);
+    }
+
+    // Kept keys (or their positions) of rows [r0, r0 + HALF_ROWS) into LDS,
+    // indexed from the first of them.
+    auto compact_keys = [&](int r0) {
+#pragma unroll
+        for (int r = r0; r < r0 + HALF_ROWS; ++r) {
+            uint32_t pos = slot[r] - row_base[r0];
+#pragma unroll
+            for (uint32_t q = 0; q < VEC; ++q) {
+                const uint32_t b = (row_flags(r) >> q) & 1u;
+                if (b) s_keys[pos] = k[r][q];
+                pos += b;
+            }
+        }
+    };
+    uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_keys);
+    auto compact_positions = [&](int r0) {
+#pragma unroll
+        for (int r = r0; r < r0 + HALF_ROWS; ++r) {
+            uint32_t pos = slot[r] - row_base[r0];
+#pragma unroll
+            for (uint32_t q = 0; q < VEC; ++q) {
+                const uint32_t b = (row_flags(r) >> q) & 1u;
+                if (b) s_pos[pos] = base + r * ROW + tid * VEC + q;
+                pos += b;
+            }
+        }
+    };
+
+    // First half into LDS while wave 0 looks back.
+    compact_keys(0);
+    if (wave == 0) {
+        if (tile == 0) {
+            if (lane == 0) s_excl = 0;
+        } else if (!s_fallback) {
+            bool gave_up = false;
+            const uint32_t ex = wave_lookback(a.status, tile, epoch, a.spin_limit, &gave_up);
+            if (lane == 0) {
+                if (gave_up) {
+                    s_fallback = 1u;
+                } else {
+                    s_excl = ex;
+                    st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, ex + block_total));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (s_fallback) {
+        const uint32_t before = reduce_before<MODE>(a, n, nseg, base, s_tmp);
+        if (tid == 0) {
+            s_excl = before;
+            st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, before + block_total));
+        }
+        __syncthreads();
+    }
+    const uint32_t tile_excl = s_excl;
+
+    T* out = static_cast<T*>(a.out0);
+    uint32_t* run_first = SEG ? nullptr : static_cast<uint32_t*>(a.out1);
+    const uint32_t h0 = row_base[HALF_ROWS];
+    for (uint32_t j = tid; j < h0; j += SCAN_BLOCK) out[tile_excl + j] = s_keys[j];
+    if (run_first) {
+        __syncthreads();
+        compact_positions(0);
+        __syncthreads();
+        for (uint32_t j = tid; j < h0; j += SCAN_BLOCK) run_first[tile_excl + j] = s_pos[j];
+    }
+    __syncthreads();
+    compact_keys(HALF_ROWS);
+    __syncthreads();
+    for (uint32_t j = tid; j < block_total - h0; j += SCAN_BLOCK) out[tile_excl + h0 + j] = s_keys[j];
+    if (run_first) {
+        __syncthreads();
+        compact_positions(HALF_ROWS);
+        __syncthreads();
+        for (uint32_t j = tid; j < block_total - h0; j += SCAN_BLOCK) run_first[tile_excl + h0 + j] = s_pos[j];
+    }
+    if (SEG) {
+        // Each cloud's base: the output index of its first key.
+        const uint32_t c_lo = s_seg[0], c_hi = s_seg[1];
+        for (uint32_t c = c_lo + tid; c < c_hi; c += SCAN_BLOCK) {
+            const uint32_t q = a.seg_off[c] - base;
+            const uint32_t v = s_chunk[(q / ROW) * SCAN_BLOCK + (q % ROW) / VEC];
+            a.seg_out[c] = tile_excl + (v & 0xFFFFu) + __popc((v >> 16) & ((1u << (q % VEC)) - 1u));
+        }
+    }
+    if (tile == num_tiles - 1 && tid == SCAN_BLOCK - 1) {
+        const uint32_t total = tile_excl + block_total;
+        if (run_first) run_first[total] = n;  // end of the last run
+        if (a.count_out) *a.count_out = total;
+    }
+}
+
 template <int MODE>
 void launch(uint32_t grid, hipStream_t s, const ScanArgs& a) {
-    hipLaunchKernelGGL(k_scan_tiles<MODE>, dim3(grid), dim3(SCAN_BLOCK), 0, s, a);
+    if constexpr (is_unique(MODE)) hipLaunchKernelGGL(k_unique_tiles<MODE>, dim3(grid), dim3(SCAN_BLOCK), 0, s, a);
+    else hipLaunchKernelGGL(k_scan_tiles<MODE>, dim3(grid), dim3(SCAN_BLOCK), 0, s, a);
 }
 
 }  // namespace
 
 uint32_t* tc_scan_epoch(tc_ctx* ctx) { return &reinterpret_cast<ScanCtl*>(ctx->small + SMALL_SCAN_CTR)->seq; }
 
-size_t tc_scan_scratch_bytes(uint32_t n) { return ((size_t)ceil_div_u32(n ? n : 1, SCAN_TILE) + 1) * 8; }
+size_t tc_scan_status_bytes() { return ((size_t)TC_SCAN_MAX_TILES + 1) * STATUS_STRIDE * sizeof(uint64_t); }
 
 int tc_persistent_grid(tc_ctx* ctx, uint32_t work_items, int block, int per_cu) {
     uint32_t blocks = ceil_div_u32(work_items ? work_items : 1, (uint32_t)block);
@@ -439,10 +693,11 @@ int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void
                    uint32_t* count_out, uint32_t n, const uint32_t* d_n_keys, const uint32_t* seg_off,
                    uint32_t n_seg_cap, const uint32_t* d_n_seg, uint32_t* seg_out, hipStream_t s,
                    bool epoch_advanced) {
-    const uint32_t tiles = ceil_div_u32(n ? n : 1, SCAN_TILE);
+    const uint32_t tile_keys = is_unique(mode) ? unique_tile_keys(mode) : SCAN_TILE;
+    const uint32_t tiles = ceil_div_u32(n ? n : 1, tile_keys);
     if (!ctx->scan_status) return TC_ERR_NOMEM;
-    // Status words for 2^18 tiles (more than 2^30 / SCAN_TILE) are preallocated with the context.
-    if (tiles > (1u << 18)) return TC_ERR_TOO_LARGE;
+    // Status words for TC_SCAN_MAX_TILES tiles (2^30 / the smallest tile) are preallocated with the context.
+    if (tiles > TC_SCAN_MAX_TILES) return TC_ERR_TOO_LARGE;
     ScanCtl* ctl = reinterpret_cast<ScanCtl*>(ctx->small + SMALL_SCAN_CTR);
     ScanArgs a{in0, in1, out0, out1, count_out, n, d_n_keys, seg_off, n_seg_cap, d_n_seg, seg_out,
                ctx->scan_status, ctl, ctx->spin_limit};

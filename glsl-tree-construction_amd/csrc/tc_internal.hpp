@@ -141,7 +141,9 @@ constexpr SortShape SORT_SHAPES[N_SORT_SHAPES] = {
 SortShape tc_sort_shape(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n);
 // The same restricted to 8-bit plans (the segmented batch sort's passes).
 SortShape tc_sort_shape_8bit(const tc_ctx* ctx, int key_bytes, bool pairs, uint32_t n);
-size_t tc_scan_scratch_bytes(uint32_t n);
+// Scan status words: one 64-byte line per tile, for up to TC_SCAN_MAX_TILES tiles.
+constexpr uint32_t TC_SCAN_MAX_TILES = 1u << 17;
+size_t tc_scan_status_bytes();
 // Looks up this device's address of the no-alloc sort's static scratch
 // (current device = ctx->device); TC_ERR_HIP if the lookup fails.
 int tc_noalloc_init(tc_ctx* ctx);

@@ -144,3 +144,22 @@ def test_batch_lookback_fallback_path(gpu_ctx, oracle, tclib):
         b.close()
     finally:
         assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 1 << 15) == 0
+
+
+def test_batch_clouds_at_unique_tile_edges(gpu_ctx, oracle, tclib):
+    """Clouds whose starts fall on and around the segmented unique's 16384-key
+    tile edges, empty clouds among them, and a stretch of 3000 clouds of 0-5
+    points (hundreds of cloud starts in one tile), against the oracle."""
+    import tc
+
+    r = np.random.default_rng(99)
+    tiny = r.integers(0, 6, size=3000).tolist()
+    sizes = [16384, 16383, 1, 16385, 0, 8192, 8191, 2, 0, 0, 16384, 40_000] + tiny + [16384, 3]
+    clouds = make_clouds(tclib, sizes, 777)
+    b = tc.Batch(gpu_ctx, len(sizes), sum(sizes), 30)
+    try:
+        run_batch(b, clouds)
+        for i, (g, pts) in enumerate(zip(b.download(), clouds)):
+            check_cloud(oracle, g, pts, 0.0, 1024.0, 30, f"cloud {i} ({len(pts)} points)")
+    finally:
+        b.close()

@@ -482,11 +482,12 @@ def test_markers_do_not_change_results(gpu_ctx, oracle, tclib):
     assert np.array_equal(got["nodes"].view(np.uint8), want["nodes"].view(np.uint8))
 
 
-@pytest.mark.parametrize("n", [1, 2, 8191, 8192, 8193, 16383, 16384, 16385, 3 * 16384 + 5, 300_001])
+@pytest.mark.parametrize("n", [1, 2, 8191, 8192, 8193, 16383, 16384, 16385, 32767, 32768, 32769, 3 * 16384 + 5,
+                               300_001])
 def test_scan_modes_at_tile_edges(gpu_ctx, oracle, n):
     """Unique (u32, u64), exclusive scan and the fused edge-count scan at sizes
-    around multiples of the scan kernels' 8192-element tile, against the
-    oracle."""
+    around multiples of the scan kernels' 8192-element tile and of unique's
+    16384-key (u32) and 8192-key (u64) tiles, against the oracle."""
     r = np.random.default_rng(n + 17)
     for dt in (np.uint32, np.uint64):
         keys = np.sort(r.integers(0, max(n // 3, 2), size=n, dtype=np.uint64).astype(dt))
@@ -509,6 +510,38 @@ def test_scan_modes_at_tile_edges(gpu_ctx, oracle, n):
             want_ec = oracle.edge_count(t["prefix_n"], t["parent"])
             assert np.array_equal(host(ec, np.uint32), want_ec)
             assert np.array_equal(host(offs, np.uint32), oracle.exclusive_scan(want_ec))
+
+
+@pytest.mark.parametrize("dt", [np.uint32, np.uint64])
+@pytest.mark.parametrize("pattern", ["one_run", "distinct", "pairs", "runs_across_tiles"])
+def test_unique_key_patterns_and_unaligned_input(gpu_ctx, oracle, dt, pattern):
+    """Unique over 5 tiles and a bit of keys in patterns that put run starts at
+    every slot, at none, and across tile and row edges; the same keys starting
+    one element into the buffer (not 16-byte aligned, so every tile takes the
+    per-key load path) must give the same result."""
+    n = 5 * 16384 + 77
+    r = np.random.default_rng(31)
+    if pattern == "one_run":
+        keys = np.full(n, 12345, dt)
+    elif pattern == "distinct":
+        keys = np.sort(r.choice(np.arange(1 << 28, dtype=np.uint64), size=n, replace=False)).astype(dt)
+    elif pattern == "pairs":
+        keys = (np.arange(n, dtype=np.uint64) // 2).astype(dt)
+    else:  # runs of 3000-20000 keys, longer than a row and than a tile
+        lens = r.integers(3000, 20_000, size=n // 3000 + 2)
+        keys = np.repeat(np.arange(len(lens), dtype=np.uint64) * 7, lens)[:n].astype(dt)
+    want, want_first = oracle.unique_runs(keys)
+    for offset in (0, 1):
+        buf = dev(np.concatenate([np.zeros(offset, dt), keys]))
+        src = buf[offset:]
+        out, first, cnt = empty(n, dt), empty(n + 1, np.uint32), empty(1, np.uint32)
+        gpu_ctx.unique(src, out, cnt, n=n)
+        u = int(host(cnt, np.uint32)[0])
+        assert u == len(want) and np.array_equal(host(out, dt, u), want), offset
+        cnt.zero_()
+        gpu_ctx.unique_runs(src, out, first, cnt, n=n)
+        assert int(host(cnt, np.uint32)[0]) == len(want)
+        assert np.array_equal(host(out, dt, u), want) and np.array_equal(host(first, np.uint32, u + 1), want_first)
 
 
 @pytest.mark.parametrize("block,kpt", [(1024, 8), (1024, 16)])
