@@ -354,16 +354,19 @@ __global__ __launch_bounds__(SEG_BLOCK, SEG_BRT_PER_CU) void k_seg_brt(Plan p, c
         }
         __syncthreads();
         const uint32_t nlong = s_nlong;
-        for (uint32_t q = threadIdx.x; q < nlong; q += SEG_BLOCK) {
+        constexpr uint32_t G = BRT_LONG_GROUP;  // lanes per long node
+        for (uint32_t q = threadIdx.x / G; q < nlong; q += SEG_BLOCK / G) {
             const uint32_t g = s_long[q];
             uint32_t c, b, nk;
             cloud_of(g, c, b, nk);
             // The raw window holds real codes for every index of the cloud inside it.
             const CodeView<K> view{uniq + b, as_lds<K>(s_win), (int32_t)(ws - (int64_t)b), (uint32_t)W};
             BrtResult r;
-            brt_node<K>(view, nk, g - b, true, &r);
-            brt_store(all, g - b, r, b);
-            if (to_clouds) brt_store(cloud_arrays(c), g - b, r);
+            brt_node_group<K, G>(view, nk, g - b, true, &r);
+            if (threadIdx.x % G == 0) {
+                brt_store(all, g - b, r, b);
+                if (to_clouds) brt_store(cloud_arrays(c), g - b, r);
+            }
         }
     }
 }

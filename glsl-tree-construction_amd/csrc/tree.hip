@@ -39,8 +39,11 @@ TC_DEV uint32_t resolve_nodes(uint32_t n_brt_cap, const uint32_t* d_n_keys) {
 // Radix tree over tiles of BRT_TILE nodes.  The workgroup copies the codes of
 // its tile plus BRT_HALO on each side into LDS; every node first runs the
 // search bounded to ranges below BRT_LIMIT keys, whose probes all fall inside
-// the window (LDS reads only, short loops).  The few nodes with longer ranges
-// are collected and searched in full afterwards, by the whole workgroup.
+// the window (LDS reads only, a lane's four nodes interleaved).  The few nodes with longer ranges
+// (about 20 per tile) are collected and searched in full afterwards, each by a
+// group of BRT_LONG_GROUP lanes: searched one per lane, their chains of
+// dependent global probes took longer than the rest of the tile (2^26
+// Gaussian-mixture build: median 12 us of a 22 us tile).
 template <typename K>
 __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
     const K* __restrict__ codes, uint32_t n_keys_cap, const uint32_t* __restrict__ d_n_keys,
@@ -66,14 +69,23 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
         if (threadIdx.x == 0) s_nlong = 0;
         stage_codes<TREE_BLOCK, BRT_TILE + 2 * BRT_HALO>(s_win, codes, ws, 0, nk, (K)~(K)0);
         __syncthreads();
-#pragma unroll 1
-        for (int k = 0; k < BRT_TILE / TREE_BLOCK; ++k) {
+        // The lane's nodes, searched together; slots past the last node
+        // search the tile's first node (always a node) and are not stored.
+        constexpr int NPL = BRT_TILE / TREE_BLOCK;
+        uint32_t idx[NPL];
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) {
+            const int64_t i = t0 + k * TREE_BLOCK + threadIdx.x;
+            idx[k] = (uint32_t)(i < nb ? i : t0);
+        }
+        BrtResult r[NPL];
+        const uint32_t near = brt_nodes_near<K, BRT_LIMIT, NPL>(as_lds<K>(s_win), (int32_t)ws, idx, ec != nullptr, r);
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) {
             const int64_t i = t0 + k * TREE_BLOCK + threadIdx.x;
             if (i >= nb) continue;
-            BrtResult r;
-            if (brt_node_near<K, BRT_LIMIT, false>(as_lds<K>(s_win), (int32_t)ws, (uint32_t)nk, (uint32_t)i,
-                                                   ec != nullptr, &r))
-                brt_store(out, (uint32_t)i, r);
+            if ((near >> k) & 1u)
+                brt_store(out, (uint32_t)i, r[k]);
             else
                 s_long[atomicAdd(&s_nlong, 1u)] = (uint32_t)i;
         }
@@ -82,10 +94,11 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
         const int64_t w1 = ws + BRT_TILE + 2 * BRT_HALO < nk ? ws + BRT_TILE + 2 * BRT_HALO : nk;
         const CodeView<K> view{codes, as_lds<K>(s_win + (w0 - ws)), (int32_t)w0, (uint32_t)(w1 - w0)};
         const uint32_t nlong = s_nlong;
-        for (uint32_t q = threadIdx.x; q < nlong; q += TREE_BLOCK) {
+        constexpr uint32_t G = BRT_LONG_GROUP;
+        for (uint32_t q = threadIdx.x / G; q < nlong; q += TREE_BLOCK / G) {
             BrtResult r;
-            brt_node<K>(view, (uint32_t)nk, s_long[q], ec != nullptr, &r);
-            brt_store(out, s_long[q], r);
+            brt_node_group<K, G>(view, (uint32_t)nk, s_long[q], ec != nullptr, &r);
+            if (threadIdx.x % G == 0) brt_store(out, s_long[q], r);
         }
     }
 }

@@ -247,6 +247,182 @@ TC_DEV bool brt_node(View codes, uint32_t nk, uint32_t i, bool want_ec, BrtResul
     return true;
 }
 
+// brt_node for one node by a group of G lanes (G divides 64; every lane of the
+// group passes the same i and gets the same result).  Each round of a search
+// probes G positions at once: the doubling step tests G powers of two, the two
+// binary searches become (G + 1)-ary.  With strictly increasing codes
+// delta(i, at(x)) does not increase with x, so every search predicate is true
+// up to a boundary and false after it, and each search returns the boundary
+// the one-lane search finds (the largest x whose probe is true).  A node whose
+// range spans 2^22 keys waits for 3 + 7 + 7 rounds of probes instead of
+// 22 + 22 + 22, which is what the nodes near the root of a cluster spend.
+// Lanes per long-range node in the radix-tree kernels.
+constexpr int BRT_LONG_GROUP = 8;
+
+template <typename K, int G, typename View = CodeView<K>>
+TC_DEV void brt_node_group(View codes, uint32_t nk, uint32_t i, bool want_ec, BrtResult* out) {
+    static_assert(G >= 2 && G <= 32 && (64 % G) == 0, "groups tile the wave");
+    const uint32_t lane = lane_id();
+    const uint32_t gl = lane % G;         // lane within the group
+    const uint32_t gshift = lane - gl;    // the group's first lane
+    const uint32_t gmask = (1u << G) - 1u;
+    auto group_bits = [&](bool p) { return (uint32_t)(__ballot(p) >> gshift) & gmask; };
+    const Delta<K, View> delta{codes, nk, codes[i]};
+    const bool fwd = delta(i + 1) - delta(i - 1) >= 0;
+    auto at = [&](uint32_t x) { return fwd ? i + x : i - x; };
+    const int dmin = delta(fwd ? i - 1 : i + 1);
+    // lmax = 2^e for the first e >= 1 with delta(at(2^e)) <= dmin (e <= 31:
+    // 2^31 > nk, outside the keys).
+    uint32_t e0 = 1, efail;
+    while (true) {
+        const uint32_t e = e0 + gl;
+        const uint32_t m = group_bits(e <= 31 && delta(at(1u << e)) > dmin);
+        if (m != gmask) {
+            efail = e0 + (uint32_t)__builtin_ctz(~m);
+            break;
+        }
+        e0 += G;
+    }
+    // Largest x in [lo, hi) whose probe is true, given it is true at lo and
+    // false at hi.
+    auto search = [&](uint32_t lo, uint32_t hi, int thr) {
+        while (hi - lo > 1) {
+            const uint32_t step = (hi - lo + G) / (G + 1);
+            const uint32_t x = lo + (gl + 1) * step;
+            const uint32_t m = group_bits(x < hi && delta(at(x)) > thr);
+            lo += (uint32_t)__builtin_ctz(~m) * step;  // the probes true below the first false one
+            hi = lo + step < hi ? lo + step : hi;
+        }
+        return lo;
+    };
+    const uint32_t l = search(0, 1u << efail, dmin);
+    const uint32_t j = at(l);
+    const int dnode = delta(j);
+    const uint32_t s = search(0, l, dnode);
+    BrtResult r;
+    r.dnode = dnode;
+    r.gamma = fwd ? i + s : i - s - 1;
+    const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+    r.leaf_l = lo == r.gamma;
+    r.leaf_r = hi == r.gamma + 1;
+    r.ec_l = r.ec_r = 0;
+    if (want_ec) {
+        const int level = dnode / 3;
+        if (!r.leaf_l) {
+            const Delta<K, View> dl{codes, nk, codes[lo]};
+            r.ec_l = (uint32_t)(dl(r.gamma) / 3 - level);
+        }
+        if (!r.leaf_r) {
+            const Delta<K, View> dr{codes, nk, codes[r.gamma + 1]};
+            r.ec_r = (uint32_t)(dr(hi) / 3 - level);
+        }
+    }
+    *out = r;
+}
+
+// NPL radix-tree nodes per lane for ranges below LIMIT keys (a power of two),
+// every probe in an LDS window (w[j - w0] = codes[j] for |j - idx[n]| < 2
+// LIMIT, slots outside the keys holding the all-ones sentinel, as for
+// brt_node_near).  The searches have fixed lengths so the NPL nodes run
+// interleaved, NPL independent LDS reads per step: the doubling step probes
+// 2, 4, ..., LIMIT at once (the first false one is where the one-by-one
+// doubling stops), and each binary search takes log2(LIMIT) bounded steps
+// (the largest x below the bound whose probe is true, as the halving and the
+// ceil(l / 2^k) steps of brt_node find).  A node's chain is 1 + 2 log2(LIMIT)
+// dependent reads, and the NPL chains overlap.  Returns bit n set for node n
+// resolved here; the others reach LIMIT keys (or are not nodes: the caller
+// masks them) and take the full search.
+template <typename K, uint32_t LIMIT, int NPL>
+TC_DEV uint32_t brt_nodes_near(lds_ptr<K> w, int32_t w0, const uint32_t (&idx)[NPL], bool want_ec,
+                               BrtResult (&res)[NPL]) {
+    static_assert(LIMIT >= 2 && (LIMIT & (LIMIT - 1)) == 0, "LIMIT is a power of two");
+    constexpr int B = CodeTraits<K>::BITS;
+    constexpr int LOG = __builtin_ctz(LIMIT);
+    auto code = [&](uint32_t j) { return w[(int32_t)j - w0]; };
+    auto dlt = [&](K a, K b) {  // -1 for "outside" (sentinel)
+        const int d = CodeTraits<K>::delta_bits(a ^ b);
+        return d > -1 ? d : -1;
+    };
+    auto thr = [&](int t) { return t >= B ? (K)0 : (K)((K)1 << (B - 1 - t)); };  // delta > t  <=>  x < thr(t)
+    K ci[NPL], tmin[NPL], tnode[NPL];
+    bool fwd[NPL];
+    uint32_t lmax[NPL], l[NPL], s[NPL], j[NPL];
+    int dnode[NPL];
+    uint32_t resolved = 0;
+#pragma unroll
+    for (int n = 0; n < NPL; ++n) {
+        const uint32_t i = idx[n];
+        ci[n] = code(i);
+        const int dl = dlt(ci[n], code(i - 1));  // slot -1 holds the sentinel
+        const int dr = dlt(ci[n], code(i + 1));
+        fwd[n] = dr >= dl;
+        tmin[n] = thr(fwd[n] ? dl : dr);
+    }
+    auto probe = [&](int n, uint32_t x, K t) {
+        const uint32_t at = fwd[n] ? idx[n] + x : idx[n] - x;
+        return (K)(ci[n] ^ code(at)) < t;
+    };
+#pragma unroll
+    for (int n = 0; n < NPL; ++n) {
+        int e = LOG + 1;
+#pragma unroll
+        for (int k = LOG; k >= 1; --k)
+            if (!probe(n, 1u << k, tmin[n])) e = k;
+        resolved |= (e <= LOG ? 1u : 0u) << n;
+        lmax[n] = e <= LOG ? 1u << e : 1u;  // unresolved nodes search nothing more here
+        l[n] = 0;
+    }
+    // Branch-free steps (a probe past the bound reads slot 0 of the search,
+    // the node itself, and is discarded), so the NPL reads of a step issue
+    // together instead of each waiting behind a branch.
+#pragma unroll
+    for (uint32_t t = LIMIT / 2; t >= 1; t /= 2)
+#pragma unroll
+        for (int n = 0; n < NPL; ++n) {
+            const uint32_t x = l[n] + t;
+            const bool in = x < lmax[n];
+            const bool p = probe(n, in ? x : 0u, tmin[n]);
+            l[n] = (in && p) ? x : l[n];
+        }
+#pragma unroll
+    for (int n = 0; n < NPL; ++n) {
+        j[n] = fwd[n] ? idx[n] + l[n] : idx[n] - l[n];
+        dnode[n] = dlt(ci[n], code(j[n]));
+        tnode[n] = thr(dnode[n]);
+        s[n] = 0;
+    }
+#pragma unroll
+    for (uint32_t t = LIMIT / 2; t >= 1; t /= 2)
+#pragma unroll
+        for (int n = 0; n < NPL; ++n) {
+            const uint32_t x = s[n] + t;
+            const bool in = x < l[n];
+            const bool p = probe(n, in ? x : 0u, tnode[n]);
+            s[n] = (in && p) ? x : s[n];
+        }
+#pragma unroll
+    for (int n = 0; n < NPL; ++n) {
+        const uint32_t i = idx[n];
+        BrtResult r;
+        r.dnode = dnode[n];
+        r.gamma = fwd[n] ? i + s[n] : i - s[n] - 1;
+        const uint32_t lo = i < j[n] ? i : j[n], hi = i < j[n] ? j[n] : i;
+        r.leaf_l = lo == r.gamma;
+        r.leaf_r = hi == r.gamma + 1;
+        r.ec_l = r.ec_r = 0;
+        if (want_ec) {
+            // All four reads unconditionally (in the window), then selects.
+            const int level = dnode[n] / 3;
+            const int el = dlt(code(lo), code(r.gamma)) / 3 - level;
+            const int er = dlt(code(r.gamma + 1), code(hi)) / 3 - level;
+            r.ec_l = r.leaf_l ? 0u : (uint32_t)el;
+            r.ec_r = r.leaf_r ? 0u : (uint32_t)er;
+        }
+        res[n] = r;
+    }
+    return resolved;
+}
+
 // The radix-tree node of brt_node for ranges below LIMIT keys, with every
 // probe in an LDS window (w[j - w0] = codes[j] for |j - i| <= LIMIT).  Each
 // probe "delta(j) > t" is one XOR and one compare against a threshold: for
