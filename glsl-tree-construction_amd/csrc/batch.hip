@@ -77,7 +77,7 @@ constexpr int SEG_BRT_PER_CU = 8;
 constexpr int SEG_OCT_PER_CU = 8;
 constexpr int SEG_ITEMS = 8;  // point loads in flight per lane in the Morton kernel
 constexpr int SEG_BRT_TILE = 1024;  // slots per workgroup step of the tree kernels
-constexpr int SEG_BRT_HALO = 512;   // LDS window halo of the octree walk
+constexpr int SEG_OCT_HALO = 128;   // LDS window halo of the octree walk
 constexpr int SEG_SEARCH_HALO = 128;     // LDS window halo of the radix-tree searches
 constexpr uint32_t SEG_BRT_LIMIT = 64;  // radix ranges searched in LDS only (within SEG_SEARCH_HALO)
 
@@ -381,9 +381,12 @@ __global__ __launch_bounds__(SEG_BLOCK, SEG_OCT_PER_CU) void k_seg_octree(const 
                                                           int link_leaves, Plan p) {
     // Tiles of slots with the links of the tile and a halo staged in LDS (see
     // k_octree_build); the window is a cache of the concatenated arrays.
-    __shared__ int32_t s_left[SEG_BRT_TILE + 2 * SEG_BRT_HALO];
-    __shared__ uint16_t s_meta[SEG_BRT_TILE + 2 * SEG_BRT_HALO];
-    __shared__ uint32_t s_list[SEG_BRT_TILE];
+    constexpr int W = SEG_BRT_TILE + 2 * SEG_OCT_HALO;
+    __shared__ int32_t s_left[W];
+    __shared__ uint16_t s_meta[W];
+    __shared__ K s_code[W];
+    __shared__ uint32_t s_off[W];
+    __shared__ uint16_t s_list[SEG_BRT_TILE];  // emitting slots, tile-relative
     __shared__ uint32_t s_tmp[SEG_BLOCK / 64 + 2];
     __shared__ uint32_t s_c0, s_b0, s_e0, s_nb0;
     __shared__ int s_rl0;
@@ -393,8 +396,8 @@ __global__ __launch_bounds__(SEG_BLOCK, SEG_OCT_PER_CU) void k_seg_octree(const 
     const uint32_t tiles = ceil_div_u32(total, SEG_BRT_TILE);
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const int64_t t0 = (int64_t)t * SEG_BRT_TILE;
-        const int64_t w0 = t0 > SEG_BRT_HALO ? t0 - SEG_BRT_HALO : 0;
-        const int64_t w1 = t0 + SEG_BRT_TILE + SEG_BRT_HALO < total ? t0 + SEG_BRT_TILE + SEG_BRT_HALO : total;
+        const int64_t w0 = t0 > SEG_OCT_HALO ? t0 - SEG_OCT_HALO : 0;
+        const int64_t w1 = t0 + SEG_BRT_TILE + SEG_OCT_HALO < total ? t0 + SEG_BRT_TILE + SEG_OCT_HALO : total;
         __syncthreads();
         if (threadIdx.x < 64) {
             // The tile's first cloud: slot range, node base, root level, quantiser.
@@ -410,7 +413,8 @@ __global__ __launch_bounds__(SEG_BLOCK, SEG_OCT_PER_CU) void k_seg_octree(const 
                 s_range0 = p.range_c[c];
             }
         }
-        stage_links<SEG_BLOCK, SEG_BRT_TILE + 2 * SEG_BRT_HALO>(s_left, s_meta, left, ec, hll, hlr, w0, w1);
+        stage_tree<SEG_BLOCK, W, K>(s_left, s_meta, s_code, s_off, left, ec, hll, hlr, prefix_n, uniq, offsets, w0,
+                                    w1);
         __syncthreads();
         const uint32_t c0 = s_c0;
         const bool to_clouds = (p.desc[3] & PLAN_OUT_PTRS) != 0;
@@ -423,21 +427,24 @@ __global__ __launch_bounds__(SEG_BLOCK, SEG_OCT_PER_CU) void k_seg_octree(const 
                 p.out[c].offsets[g - ubase[c]] = offsets[g] - node_base[c];
             }
         }
-        // Compact the slots with ec > 0 (the last slot of a cloud has ec 0).
-        uint32_t count = 0;
-#pragma unroll 1
-        for (int k = 0; k < SEG_BRT_TILE / SEG_BLOCK; ++k) {
-            const int64_t g = t0 + k * SEG_BLOCK + threadIdx.x;
-            const uint32_t flag = (g < total && (s_meta[g - w0] & 255u) != 0u) ? 1u : 0u;
-            uint32_t tot;
-            const uint32_t rank = block_exclusive_scan<SEG_BLOCK>(flag, s_tmp, &tot);
-            if (flag) s_list[count + rank] = (uint32_t)g;
-            count += tot;
+        // Compact the slots with ec > 0 (the last slot of a cloud has ec 0);
+        // thread t looks at slots 4t .. 4t + 3 of the tile.
+        constexpr int NPT = SEG_BRT_TILE / SEG_BLOCK;
+        uint32_t flags = 0;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int64_t g = t0 + NPT * threadIdx.x + k;
+            flags |= ((g < total && (s_meta[g - w0] & 255u) != 0u) ? 1u : 0u) << k;
         }
+        uint32_t count;
+        uint32_t rank = block_exclusive_scan<SEG_BLOCK>(__popc(flags), s_tmp, &count);
+#pragma unroll
+        for (int k = 0; k < NPT; ++k)
+            if ((flags >> k) & 1u) s_list[rank++] = (uint16_t)(NPT * threadIdx.x + k);
         __syncthreads();
 #pragma unroll 1
         for (uint32_t q = threadIdx.x; q < count; q += SEG_BLOCK) {
-            const uint32_t g = s_list[q];
+            const uint32_t g = (uint32_t)t0 + s_list[q];
             // Cloud data from LDS while g is in the tile's first cloud.
             uint32_t c = s_c0, b = s_b0, e = s_e0, nbase = s_nb0;
             int root_level = s_rl0;
@@ -454,13 +461,14 @@ __global__ __launch_bounds__(SEG_BLOCK, SEG_OCT_PER_CU) void k_seg_octree(const 
             const uint32_t nk = e - b;
             const uint32_t i = g - b;
             if (i + 1 >= nk) continue;
-            const LinkView lv{left, hll, hlr, ec, as_lds<int32_t>(s_left), as_lds<uint16_t>(s_meta),
-                              (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0), b};
+            const TreeView<K> lv{left, hll, hlr, ec, prefix_n, uniq, offsets, as_lds<int32_t>(s_left),
+                                 as_lds<uint16_t>(s_meta), as_lds<K>(s_code), as_lds<uint32_t>(s_off),
+                                 (int32_t)(w0 - (int64_t)b), (uint32_t)(w1 - w0), b};
             // Nodes go to the cloud's own buffer when there is one (then only there).
             tc_oct_node* dst = to_clouds ? p.out[c].nodes : nodes;
             const uint32_t cap = to_clouds ? d_octree_node_bound(nk, CodeTraits<K>::BITS) : node_cap;
-            octree_build_node<K>(dst, cap, nbase, offsets, ec, uniq, prefix_n, lv, i, nk - 1, root_level,
-                                 link_leaves, mn, rg, to_clouds ? nbase : 0u);
+            octree_build_node<K>(dst, cap, nbase, lv, i, nk - 1, root_level, link_leaves, mn, rg,
+                                 to_clouds ? nbase : 0u);
         }
     }
 }

@@ -478,29 +478,36 @@ TC_DEV uint64_t d_encode63(float x, float y, float z, float min_coord, float ran
 template <typename K> struct CodeTraits;
 template <> struct CodeTraits<uint32_t> {
     static constexpr int BITS = 30;
+    static constexpr int AXIS_BITS = 10;
     TC_DEV static int delta_bits(uint32_t x) { return x ? __clz(x) - 2 : 30; }
+    // The cell corner of integer coordinates q (code = interleave(q)).
+    TC_DEV static float corner(uint32_t q, float min_coord, float range) {
+        float f = (float)q;
+        f = __fmul_rn(f, 0x1p-10f);  // exactly f / 1024: a power-of-two scale
+        f = __fmul_rn(f, range);
+        return __fadd_rn(f, min_coord);
+    }
+    TC_DEV static uint32_t axis(uint32_t code, int a) { return d_compact3_10(code >> a); }
     TC_DEV static void decode(uint32_t code, float min_coord, float range, float* out) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float f = (float)d_compact3_10(code >> a);
-            f = __fmul_rn(f, 0x1p-10f);  // exactly f / 1024: a power-of-two scale
-            f = __fmul_rn(f, range);
-            out[a] = __fadd_rn(f, min_coord);
-        }
+        for (int a = 0; a < 3; ++a) out[a] = corner(axis(code, a), min_coord, range);
         out[3] = 1.0f;
     }
 };
 template <> struct CodeTraits<uint64_t> {
     static constexpr int BITS = 63;
+    static constexpr int AXIS_BITS = 21;
     TC_DEV static int delta_bits(uint64_t x) { return x ? __clzll(x) - 1 : 63; }
+    TC_DEV static float corner(uint32_t q, float min_coord, float range) {
+        float f = (float)q;
+        f = __fmul_rn(f, 0x1p-21f);  // exactly f / 2^21
+        f = __fmul_rn(f, range);
+        return __fadd_rn(f, min_coord);
+    }
+    TC_DEV static uint32_t axis(uint64_t code, int a) { return (uint32_t)d_compact3_21(code >> a); }
     TC_DEV static void decode(uint64_t code, float min_coord, float range, float* out) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float f = (float)d_compact3_21(code >> a);
-            f = __fmul_rn(f, 0x1p-21f);  // exactly f / 2^21
-            f = __fmul_rn(f, range);
-            out[a] = __fadd_rn(f, min_coord);
-        }
+        for (int a = 0; a < 3; ++a) out[a] = corner(axis(code, a), min_coord, range);
         out[3] = 1.0f;
     }
 };

@@ -125,42 +125,50 @@ __global__ __launch_bounds__(TREE_BLOCK, 8) void k_octree_build(
     const uint32_t* __restrict__ ec, const K* __restrict__ codes, const uint8_t* __restrict__ prefix_n,
     const int32_t* __restrict__ left_child, const uint8_t* __restrict__ hll, const uint8_t* __restrict__ hlr,
     uint32_t n_brt_cap, const uint32_t* __restrict__ d_n_keys, int link_leaves, float min_coord, float range) {
-    __shared__ int32_t s_left[OCT_TILE + 2 * OCT_HALO];
-    __shared__ uint16_t s_meta[OCT_TILE + 2 * OCT_HALO];
-    __shared__ uint32_t s_list[OCT_TILE];
+    constexpr int W = OCT_TILE + 2 * OCT_HALO;
+    __shared__ int32_t s_left[W];
+    __shared__ uint16_t s_meta[W];
+    __shared__ K s_code[W];
+    __shared__ uint32_t s_off[W];
+    __shared__ uint16_t s_list[OCT_TILE];  // emitting nodes, tile-relative
     __shared__ uint32_t s_tmp[TREE_BLOCK / 64 + 2];
     const uint32_t nb = resolve_nodes(n_brt_cap, d_n_keys);
     if (nb == 0) return;
     const int root_level = prefix_n[0] / 3;
     const uint32_t tiles = ceil_div_u32(nb, OCT_TILE);
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        // Stage the links of the tile and a halo: the walks below a node stay
-        // inside its key range, so nearly all of their reads hit LDS.
+        // Stage the tree arrays of the tile and a halo: the walks below a node
+        // stay inside its key range, so nearly all of their reads hit LDS.
         const int64_t t0 = (int64_t)t * OCT_TILE;
         const int64_t w0 = t0 > OCT_HALO ? t0 - OCT_HALO : 0;
         const int64_t w1 = t0 + OCT_TILE + OCT_HALO < nb ? t0 + OCT_TILE + OCT_HALO : nb;
         __syncthreads();
-        stage_links<TREE_BLOCK, OCT_TILE + 2 * OCT_HALO>(s_left, s_meta, left_child, ec, hll, hlr, w0, w1);
+        stage_tree<TREE_BLOCK, W, K>(s_left, s_meta, s_code, s_off, left_child, ec, hll, hlr, prefix_n, codes,
+                                     offsets, w0, w1);
         __syncthreads();
         // Only nodes with ec > 0 emit anything (typically well under half):
-        // compact them so that every lane of a wave has one.
-        uint32_t count = 0;
-#pragma unroll 1
-        for (int k = 0; k < OCT_TILE / TREE_BLOCK; ++k) {
-            const int64_t i = t0 + k * TREE_BLOCK + threadIdx.x;
-            const uint32_t flag = (i < nb && (s_meta[i - w0] & 255u) != 0u) ? 1u : 0u;
-            uint32_t tot;
-            const uint32_t rank = block_exclusive_scan<TREE_BLOCK>(flag, s_tmp, &tot);
-            if (flag) s_list[count + rank] = (uint32_t)i;
-            count += tot;
+        // compact them so that every lane of a wave has one.  Thread t looks
+        // at nodes 4t .. 4t + 3 of the tile: one block scan.
+        constexpr int NPT = OCT_TILE / TREE_BLOCK;
+        uint32_t flags = 0;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int64_t i = t0 + NPT * threadIdx.x + k;
+            flags |= ((i < nb && (s_meta[i - w0] & 255u) != 0u) ? 1u : 0u) << k;
         }
+        uint32_t count;
+        uint32_t rank = block_exclusive_scan<TREE_BLOCK>(__popc(flags), s_tmp, &count);
+#pragma unroll
+        for (int k = 0; k < NPT; ++k)
+            if ((flags >> k) & 1u) s_list[rank++] = (uint16_t)(NPT * threadIdx.x + k);
         __syncthreads();
-        const LinkView lv{left_child, hll, hlr, ec, as_lds<int32_t>(s_left), as_lds<uint16_t>(s_meta), (int32_t)w0,
-                          (uint32_t)(w1 - w0)};
+        const TreeView<K> lv{left_child, hll, hlr, ec, prefix_n, codes, offsets, as_lds<int32_t>(s_left),
+                             as_lds<uint16_t>(s_meta), as_lds<K>(s_code), as_lds<uint32_t>(s_off), (int32_t)w0,
+                             (uint32_t)(w1 - w0)};
 #pragma unroll 1
         for (uint32_t q = threadIdx.x; q < count; q += TREE_BLOCK)
-            octree_build_node<K>(nodes, node_cap, 0u, offsets, ec, codes, prefix_n, lv, s_list[q], nb, root_level,
-                                 link_leaves, min_coord, range);
+            octree_build_node<K>(nodes, node_cap, 0u, lv, (uint32_t)t0 + s_list[q], nb, root_level, link_leaves,
+                                 min_coord, range);
     }
 }
 

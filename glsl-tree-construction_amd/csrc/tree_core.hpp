@@ -35,26 +35,36 @@ TC_DEV lds_ptr<K> as_lds(const K* p) {
     return (lds_ptr<K>)p;
 }
 
-// Radix-tree links of one cloud (cloud-local indices), read through an
-// optional LDS copy of the window [w0, w0 + wlen): the left child and a packed
-// word edge_count | has_leaf_left << 8 | has_leaf_right << 9 (edge counts are
-// at most 21).  The octree's walks down the radix tree read these.  Outside
-// the window, local index x is read at gb + x of the arrays (gb: the cloud's
-// first slot in concatenated arrays, else 0; the pointers stay uniform).
-struct LinkView {
+// The radix tree of one cloud (cloud-local indices) as the octree stage reads
+// it, through an optional LDS copy of the window [w0, w0 + wlen): left child,
+// code, octree node offset, and a packed meta word edge_count | has_leaf_left
+// << 8 | has_leaf_right << 9 | prefix_n << 10 (edge counts are at most 21,
+// prefix lengths at most 63).  Outside the window, local index x is read at
+// gb + x of the arrays (gb: the cloud's first slot in concatenated arrays,
+// else 0; the pointers stay uniform).
+template <typename K>
+struct TreeView {
     const int32_t* __restrict__ left;
     const uint8_t* __restrict__ hll;
     const uint8_t* __restrict__ hlr;
     const uint32_t* __restrict__ ec;
+    const uint8_t* __restrict__ prefix_n;
+    const K* __restrict__ codes;
+    const uint32_t* __restrict__ offsets;
     lds_ptr<int32_t> wleft = nullptr;
     lds_ptr<uint16_t> wmeta = nullptr;
+    lds_ptr<K> wcode = nullptr;
+    lds_ptr<uint32_t> woff = nullptr;
     int32_t w0 = 0;
     uint32_t wlen = 0;
     uint32_t gb = 0;
 
-    static TC_DEV uint16_t pack(uint32_t e, uint8_t l, uint8_t r) {
-        return (uint16_t)((e < 255u ? e : 255u) | ((uint32_t)(l != 0) << 8) | ((uint32_t)(r != 0) << 9));
+    static TC_DEV uint16_t pack(uint32_t e, uint8_t l, uint8_t r, uint8_t pn) {
+        return (uint16_t)((e < 255u ? e : 255u) | ((uint32_t)(l != 0) << 8) | ((uint32_t)(r != 0) << 9) |
+                          ((uint32_t)(pn & 63u) << 10));
     }
+    static TC_DEV uint32_t edge_count(uint32_t m) { return m & 255u; }
+    static TC_DEV int prefix(uint32_t m) { return (int)(m >> 10); }
     TC_DEV int32_t left_child(uint32_t x) const {
         const uint32_t r = x - (uint32_t)w0;
         if (r < wlen) return wleft[r];
@@ -63,7 +73,17 @@ struct LinkView {
     TC_DEV uint32_t meta(uint32_t x) const {
         const uint32_t r = x - (uint32_t)w0;
         if (r < wlen) return wmeta[r];
-        return pack(ec[gb + x], hll[gb + x], hlr[gb + x]);
+        return pack(ec[gb + x], hll[gb + x], hlr[gb + x], prefix_n[gb + x]);
+    }
+    TC_DEV K code(uint32_t x) const {
+        const uint32_t r = x - (uint32_t)w0;
+        if (r < wlen) return wcode[r];
+        return codes[gb + x];
+    }
+    TC_DEV uint32_t offset(uint32_t x) const {
+        const uint32_t r = x - (uint32_t)w0;
+        if (r < wlen) return woff[r];
+        return offsets[gb + x];
     }
 };
 
@@ -97,36 +117,49 @@ TC_DEV void stage_codes(K* dst, const K* __restrict__ src, int64_t j0, int64_t l
     }
 }
 
-// Links of entries [w0, w1) (0 < w1 - w0 <= W) packed for LinkView.  The
+// Window [w0, w1) (0 < w1 - w0 <= W) of the tree arrays for TreeView.  The
 // lane index is made opaque here: otherwise the compiler hoists the per-lane
-// part of all 4 x PER load addresses (64-bit each) out of the caller's tile
-// loop, keeps them live across it and spills them (14 VGPRs of the octree
-// kernel at eight workgroups per CU, reloaded from scratch every tile).
-template <int BLOCK, int W>
-TC_DEV void stage_links(int32_t* s_left, uint16_t* s_meta, const int32_t* __restrict__ left,
-                        const uint32_t* __restrict__ ec, const uint8_t* __restrict__ hll,
-                        const uint8_t* __restrict__ hlr, int64_t w0, int64_t w1) {
+// part of all the PER x 7 load addresses (64-bit each) out of the caller's
+// tile loop, keeps them live across it and spills them.
+template <int BLOCK, int W, typename K>
+TC_DEV void stage_tree(int32_t* s_left, uint16_t* s_meta, K* s_code, uint32_t* s_off,
+                       const int32_t* __restrict__ left, const uint32_t* __restrict__ ec,
+                       const uint8_t* __restrict__ hll, const uint8_t* __restrict__ hlr,
+                       const uint8_t* __restrict__ prefix_n, const K* __restrict__ codes,
+                       const uint32_t* __restrict__ offsets, int64_t w0, int64_t w1) {
     constexpr int PER = (W + BLOCK - 1) / BLOCK;
     int32_t vl[PER];
-    uint32_t ve[PER];
-    uint8_t va[PER], vb[PER];
+    uint32_t ve[PER], vo[PER];
+    uint8_t va[PER], vb[PER], vp[PER];
+    K vc[PER];
     uint32_t lane_t = threadIdx.x;
     asm volatile("" : "+v"(lane_t));
+    // 32-bit byte offsets from the uniform base pointers (indices are below
+    // 2^30), so each load is one base register plus one offset register.
+    auto at = [](const auto* base, uint32_t j) {
+        using T = typename std::remove_cv<typename std::remove_pointer<decltype(base)>::type>::type;
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + j * (uint32_t)sizeof(T));
+    };
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-        int64_t j = w0 + r * BLOCK + (int)lane_t;
-        j = j < w1 ? j : w1 - 1;
-        vl[r] = left[j];
-        ve[r] = ec[j];
-        va[r] = hll[j];
-        vb[r] = hlr[j];
+        int64_t jj = w0 + r * BLOCK + (int)lane_t;
+        const uint32_t j = (uint32_t)(jj < w1 ? jj : w1 - 1);
+        vl[r] = at(left, j);
+        ve[r] = at(ec, j);
+        va[r] = at(hll, j);
+        vb[r] = at(hlr, j);
+        vp[r] = at(prefix_n, j);
+        vc[r] = at(codes, j);
+        vo[r] = at(offsets, j);
     }
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         const int q = r * BLOCK + (int)lane_t;
         if (w0 + q < w1) {
             s_left[q] = vl[r];
-            s_meta[q] = LinkView::pack(ve[r], va[r], vb[r]);
+            s_meta[q] = TreeView<K>::pack(ve[r], va[r], vb[r], vp[r]);
+            s_code[q] = vc[r];
+            s_off[q] = vo[r];
         }
     }
 }
@@ -524,20 +557,17 @@ TC_DEV uint32_t child_slot(K code, int level) {
 // and the racy mask updates (each node is written once, whole, by one lane).
 // offsets[] hold global node indices; stored links are local (minus node_base).
 // Indices a, c are cloud-local; arrays are read at gb + index (lv.gb, see
-// LinkView).
+// TreeView).
 template <typename K>
 TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap, uint32_t node_base,
-                              const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ec,
-                              const K* __restrict__ codes, const uint8_t* __restrict__ prefix_n,
-                              const LinkView& lv, uint32_t a, uint32_t nb, int root_level, int link_leaves,
+                              const TreeView<K>& lv, uint32_t a, uint32_t nb, int root_level, int link_leaves,
                               float min_coord, float range, uint32_t write_base = 0) {
-    constexpr int BITS = CodeTraits<K>::BITS;
-    const uint32_t gb = lv.gb;
-    const uint32_t cnt = ec[gb + a];
+    const uint32_t ma = lv.meta(a);
+    const uint32_t cnt = TreeView<K>::edge_count(ma);
     if (cnt == 0) return;
-    const uint32_t off = offsets[gb + a];
-    const K code = codes[gb + a];
-    const int bottom = prefix_n[gb + a] / 3;
+    const uint32_t off = lv.offset(a);
+    const K code = lv.code(a);
+    const int bottom = TreeView<K>::prefix(ma) / 3;
 
     // Walk (LDS reads only while inside the window).  Candidate p's kind is
     // two bits of `kinds`: 0 none, 1 radix leaf, 2 internal node with its own
@@ -563,7 +593,7 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
                 cand[ps] = c;
                 kinds |= (link_leaves ? 1u : 0u) << (2 * ps);
             } else if (c < nb) {
-                const uint32_t e = lv.meta(c) & 255u;
+                const uint32_t e = TreeView<K>::edge_count(lv.meta(c));
                 if (e > 0 || !next) {
                     cand[ps] = c;
                     kinds |= (e > 0 ? 2u : 0u) << (2 * ps);
@@ -596,10 +626,10 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
             const uint32_t kind = (kinds >> (2 * p)) & 3u;
             if (kind == 0) continue;
             const uint32_t c = cand[p];
-            const uint32_t slot = child_slot<K>(codes[gb + c], bottom + 1);
+            const uint32_t slot = child_slot<K>(lv.code(c), bottom + 1);
             int32_t v = (int32_t)c;
             if (kind == 2) {
-                v = (int32_t)(offsets[gb + c] + (lv.meta(c) & 255u) - 1 - node_base);
+                v = (int32_t)(lv.offset(c) + TreeView<K>::edge_count(lv.meta(c)) - 1 - node_base);
                 bmask |= 1 << slot;
             }
 #pragma unroll
@@ -611,14 +641,26 @@ TC_DEV void octree_build_node(tc_oct_node* __restrict__ nodes, uint32_t node_cap
     // Node idx (global) is stored at nodes[idx - write_base] (nodes may be one
     // cloud's own buffer, of node_cap entries).  The bottom node (j = 0) takes
     // the gathered children; every chain node above has one child, the node
-    // one level deeper, whose slot and link are formed at the store.
+    // one level deeper, whose slot and link are formed at the store.  A chain
+    // node's corner is the decode of the code cut to its level; cutting the
+    // code to its top 3 L bits keeps the top L bits of each axis, so the axes
+    // are decoded once and cut per node (the same integers, the same floats).
+    constexpr int AB = CodeTraits<K>::AXIS_BITS;
+    uint32_t q[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) q[ax] = CodeTraits<K>::axis(code, ax);
     for (uint32_t j = 0; j < cnt; ++j) {
         const uint32_t idx = off + j;
         if (idx - write_base >= node_cap) break;
         const int level = bottom - (int)j;
-        const int sh = BITS - 3 * level;
+        const int cut = AB - level;  // 0 <= cut <= AB
         float c4[4];
-        CodeTraits<K>::decode((code >> sh) << sh, min_coord, range, c4);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const uint32_t qa = (q[ax] >> cut) << cut;
+            c4[ax] = CodeTraits<K>::corner(qa, min_coord, range);
+        }
+        c4[3] = 1.0f;
         // range / 2^(level - root_level), exact: a power-of-two scale.
         const float cell = ldexpf(range, root_level - level);
         int4* dst = reinterpret_cast<int4*>(nodes + (idx - write_base));
