@@ -329,22 +329,33 @@ __global__ __launch_bounds__(SEG_BLOCK, SEG_BRT_PER_CU) void k_seg_brt(Plan p, c
                 nk = ubase[c + 1] - b;
             }
         };
-#pragma unroll 1
-        for (int k = 0; k < SEG_BRT_TILE / SEG_BLOCK; ++k) {
+        // The lane's slots in the tile's first cloud, searched together (as in
+        // k_build_radix_tree); other slots search the tile's first slot (in
+        // that cloud, every probe inside the window) and discard the result.
+        constexpr int NPL = SEG_BRT_TILE / SEG_BLOCK;
+        uint32_t idx[NPL];
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) {
             const uint32_t g = (uint32_t)t0 + k * SEG_BLOCK + threadIdx.x;
-            if (g >= total) break;
+            idx[k] = (g < e0 && g < total) ? g - b0 : (uint32_t)t0 - b0;
+        }
+        BrtResult rr[NPL];
+        const uint32_t near =
+            brt_nodes_near<K, SEG_BRT_LIMIT, NPL>(as_lds<K>(s_fast), (int32_t)(ws - (int64_t)b0), idx, true, rr);
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) {
+            const uint32_t g = (uint32_t)t0 + k * SEG_BLOCK + threadIdx.x;
+            if (g >= total) continue;
             uint32_t c, b, nk;
             cloud_of(g, c, b, nk);
             const uint32_t i = g - b;
             if (to_clouds) static_cast<K*>(p.out[c].unique)[i] = s_win[g - ws];
             if (i + 1 < nk) {
-                BrtResult r;
                 // Nodes of later clouds in the tile (tiles that straddle a
                 // cloud boundary) take the full search below.
-                if (c == c0 && brt_node_near<K, SEG_BRT_LIMIT, false>(as_lds<K>(s_fast), (int32_t)(ws - (int64_t)b),
-                                                                       nk, i, true, &r)) {
-                    brt_store(all, i, r, b);
-                    if (to_clouds) brt_store(cloud_arrays(c), i, r);
+                if (c == c0 && ((near >> k) & 1u)) {
+                    brt_store(all, i, rr[k], b);
+                    if (to_clouds) brt_store(cloud_arrays(c), i, rr[k]);
                 } else {
                     s_long[atomicAdd(&s_nlong, 1u)] = g;
                 }

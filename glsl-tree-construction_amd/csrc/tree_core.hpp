@@ -353,10 +353,14 @@ TC_DEV void brt_node_group(View codes, uint32_t nk, uint32_t i, bool want_ec, Br
     *out = r;
 }
 
-// NPL radix-tree nodes per lane for ranges below LIMIT keys (a power of two),
-// every probe in an LDS window (w[j - w0] = codes[j] for |j - idx[n]| < 2
-// LIMIT, slots outside the keys holding the all-ones sentinel, as for
-// brt_node_near).  The searches have fixed lengths so the NPL nodes run
+// NPL radix-tree nodes per lane (brt_node) for ranges below LIMIT keys (a
+// power of two), every probe in an LDS window: w[j - w0] = codes[j] for
+// |j - idx[n]| < 2 LIMIT, with slots outside the cloud's keys (and slot -1)
+// holding an all-ones sentinel, whose XOR with any code is at least 2^B for
+// codes of B bits, i.e. "no common prefix".  Each probe "delta(j) > t" is one
+// XOR and one compare against a threshold: for codes of B bits in a word of W
+// bits, clz(x) - (W - B) > t exactly when x < 2^(B - 1 - t).  The searches
+// have fixed lengths so the NPL nodes run
 // interleaved, NPL independent LDS reads per step: the doubling step probes
 // 2, 4, ..., LIMIT at once (the first false one is where the one-by-one
 // doubling stops), and each binary search takes log2(LIMIT) bounded steps
@@ -454,84 +458,6 @@ TC_DEV uint32_t brt_nodes_near(lds_ptr<K> w, int32_t w0, const uint32_t (&idx)[N
         res[n] = r;
     }
     return resolved;
-}
-
-// The radix-tree node of brt_node for ranges below LIMIT keys, with every
-// probe in an LDS window (w[j - w0] = codes[j] for |j - i| <= LIMIT).  Each
-// probe "delta(j) > t" is one XOR and one compare against a threshold: for
-// codes of B bits in a word of W bits, clz(x) - (W - B) > t exactly when
-// x < 2^(B - 1 - t).  With CHECK the probe also tests j < nk (windows shared by
-// several clouds); without it, window slots outside [0, nk) must hold an
-// all-ones sentinel, whose XOR with any code is at least 2^B, i.e. "no common
-// prefix".  Returns false when the range reaches LIMIT.
-template <typename K, uint32_t LIMIT, bool CHECK>
-TC_DEV bool brt_node_near(lds_ptr<K> w, int32_t w0, uint32_t nk, uint32_t i, bool want_ec, BrtResult* out) {
-    constexpr int B = CodeTraits<K>::BITS;
-    auto code = [&](uint32_t j) { return w[(int32_t)j - w0]; };
-    auto dlt = [&](K a, K b) {  // -1 for "outside" (sentinel)
-        const int d = CodeTraits<K>::delta_bits(a ^ b);
-        return d > -1 ? d : -1;
-    };
-    auto thr = [&](int t) { return t >= B ? (K)0 : (K)((K)1 << (B - 1 - t)); };  // delta > t  <=>  x < thr(t)
-    const K ci = code(i);
-    // i >= 1 or the sentinel below 0 (CHECK: slot i's own code is read instead
-    // and the result masked, so the read needs no branch).
-    int dl;
-    if constexpr (CHECK) {
-        const K cl = code(i >= 1 ? i - 1 : i);
-        dl = i >= 1 ? dlt(ci, cl) : -1;
-    } else {
-        dl = dlt(ci, code(i - 1));
-    }
-    const int dr = dlt(ci, code(i + 1));  // i + 1 <= nk - 1: always a key
-    const bool fwd = dr >= dl;
-    auto at = [&](uint32_t x) { return fwd ? i + x : i - x; };
-    // With CHECK, a probe outside [0, nk) (i - x wraps to >= nk) reads slot i
-    // (always staged) and fails; probes inside [0, nk) lie within LIMIT of the
-    // tile, inside the window.  Branch-free: the read is not predicated.
-    auto probe = [&](uint32_t x, K t) {
-        const uint32_t j = at(x);
-        if constexpr (CHECK) {
-            const bool in = j < nk;
-            const K cj = code(in ? j : i);
-            return in && (K)(ci ^ cj) < t;
-        } else {
-            return (K)(ci ^ code(j)) < t;
-        }
-    };
-    const int dmin = fwd ? dl : dr;
-    const K tmin = thr(dmin);
-    uint32_t lmax = 2;
-    while (probe(lmax, tmin)) {
-        lmax *= 2;
-        if (lmax > LIMIT) return false;
-    }
-    uint32_t l = 0;
-    for (uint32_t t = lmax / 2; t >= 1; t /= 2)
-        if (probe(l + t, tmin)) l += t;
-    const uint32_t j = at(l);
-    const int dnode = dlt(ci, code(j));
-    const K tnode = thr(dnode);
-    uint32_t s = 0;
-    const int kmax = l <= 1 ? 0 : log2_ceil_u32(l);
-    for (int k = 1; k <= kmax; ++k) {
-        const uint32_t t = (l + (1u << k) - 1) >> k;
-        if (probe(s + t, tnode)) s += t;
-    }
-    BrtResult r;
-    r.dnode = dnode;
-    r.gamma = fwd ? i + s : i - s - 1;
-    const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
-    r.leaf_l = lo == r.gamma;
-    r.leaf_r = hi == r.gamma + 1;
-    r.ec_l = r.ec_r = 0;
-    if (want_ec) {
-        const int level = dnode / 3;
-        if (!r.leaf_l) r.ec_l = (uint32_t)(dlt(code(lo), code(r.gamma)) / 3 - level);
-        if (!r.leaf_r) r.ec_r = (uint32_t)(dlt(code(r.gamma + 1), code(hi)) / 3 - level);
-    }
-    *out = r;
-    return true;
 }
 
 template <typename K>
