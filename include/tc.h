@@ -221,7 +221,7 @@ int tc_sort_pairs_u64(tc_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* key
 /* tc_sort_keys_u32 that allocates nothing.  No reference counterpart: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * serving loops, harnesses that forbid hipMalloc).  Its scratch is a static
- * 4.26 MiB device array per GPU, held until the process ends, found at
+ * 5.03 MiB device array per GPU, held until the process ends, found at
  * tc_create; it calls no hipMalloc/hipFree.  n <= 2^24 (TC_ERR_TOO_LARGE
  * above).  Every call on one GPU shares that array, whatever the context:
  * calls that overlap on one GPU (other streams, other contexts) race and must
