@@ -122,7 +122,9 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
                                                                K* __restrict__ codes, uint32_t* __restrict__ seg_hist,
                                                                uint32_t tile_size) {
     constexpr int DIG = (int)sizeof(K);
+    constexpr bool TABLE = sizeof(K) == 4;  // 30-bit codes: bit spreading from an LDS table
     __shared__ uint32_t s_h[DIG * 256];
+    __shared__ uint32_t s_split[TABLE ? 1024 : 1];
     __shared__ uint32_t s_cloud;
     const uint32_t C = p.desc[0], T = p.desc[1];
     const uint32_t chunk = ceil_div_u32(T, gridDim.x);
@@ -130,6 +132,8 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
     const uint32_t t1 = t0 + chunk < T ? t0 + chunk : T;
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < DIG * 256; i += SEG_BLOCK) s_h[i] = 0;
+    if constexpr (TABLE)
+        for (uint32_t i = tid; i < 1024; i += SEG_BLOCK) s_split[i] = d_split3_10(i);
     uint32_t cur = 0xFFFFFFFFu;
     for (uint32_t t = t0; t < t1; ++t) {
         if (tid < 64) {
@@ -169,15 +173,38 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_morton_hist(Plan p, const flo
                 const uint32_t i = i0 + u * SEG_BLOCK;
                 if (i < end) qv[u] = ld_stream(&src[i - c0]);
             }
+            K code[SEG_ITEMS];
 #pragma unroll
             for (int u = 0; u < SEG_ITEMS; ++u) {
                 const uint32_t i = i0 + u * SEG_BLOCK;
+                code[u] = 0;
                 if (i < end) {
-                    K code;
-                    if constexpr (sizeof(K) == 4) code = d_encode30(qv[u].x, qv[u].y, qv[u].z, mn, rg);
-                    else code = d_encode63(qv[u].x, qv[u].y, qv[u].z, mn, rg);
-                    codes[i] = code;
-                    lds_count_digits<K, DIG>(s_h, code);
+                    if constexpr (TABLE) {
+                        const MortonQuant q(mn, rg, 1024.0f);
+                        code[u] = s_split[q(qv[u].x) & 1023u] | (s_split[q(qv[u].y) & 1023u] << 1) |
+                                  (s_split[q(qv[u].z) & 1023u] << 2);
+                    } else {
+                        code[u] = d_encode63(qv[u].x, qv[u].y, qv[u].z, mn, rg);
+                    }
+                    codes[i] = code[u];
+                }
+            }
+            // The wave-uniform shortcut of lds_count_digits only pays for
+            // points in spatial order: taken for this round when the first
+            // point's top digit is shared by the wave (item 0 is a point for
+            // every lane still in the loop).
+            const uint32_t top = (uint32_t)(code[0] >> (8 * (DIG - 1)));
+            const bool uni = __ballot(top == __builtin_amdgcn_readfirstlane(top)) == __ballot(true);
+#pragma unroll
+            for (int u = 0; u < SEG_ITEMS; ++u) {
+                if (i0 + u * SEG_BLOCK < end) {
+                    if (uni) {
+                        lds_count_digits<K, DIG>(s_h, code[u]);
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < DIG; ++d)
+                            atomicAdd(&s_h[d * 256 + (uint32_t)((code[u] >> (8 * d)) & 255u)], 1u);
+                    }
                 }
             }
         }
