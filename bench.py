@@ -207,6 +207,8 @@ def bench_sort_u32(ctx, codes, steps):
         keys.copy_(codes)
         ctx.sort(keys, alt)
 
+    for _ in range(3):  # warm-up: first-call costs stay out of the averages, as for the build
+        step()
     prof = profile_stages(ctx, step, steps)
     ms = prof["histogram"][0] + prof["binning"][0] * prof["binning"][1]
     return {
@@ -239,6 +241,7 @@ def bench_sort_pairs_u64(ctx, log2n, steps):
         vals.copy_(vals0)
         ctx.sort(keys, ka, vals, va)
 
+    step()  # warm-up
     prof = profile_stages(ctx, step, steps)
     ms = prof["histogram"][0] + prof["binning"][0] * prof["binning"][1]
     ok = bool(torch.all(keys[1:] >= keys[:-1]).item())
@@ -544,7 +547,7 @@ def main():
         }
         codes = torch.empty(n, dtype=torch.int32, device="cuda")
         ctx.morton30(pts, codes, 0.0, 1024.0)
-        out["sort"] = bench_sort_u32(ctx, codes, args.profile_steps)
+        out["sort"] = bench_sort_u32(ctx, codes, 20)
         del codes
         del pts
         torch.cuda.empty_cache()

@@ -66,3 +66,27 @@ def load(ctx, path: str):
 
 def clear(ctx):
     check(lib().tc_clear_sort_tuning(ctx.handle), "tc_clear_sort_tuning")
+
+
+def main(argv=None) -> int:
+    """Re-time every built shape on device 0 and write the table:
+    ``python -m tc.tune --out tc/tuning_mi355x.json --max-log2n 26`` (run from
+    the package directory)."""
+    import argparse
+
+    from .api import Context
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--max-log2n", type=int, default=26)
+    args = ap.parse_args(argv)
+    ctx = Context(0)
+    autotune(ctx, 1 << args.max_log2n)
+    save(ctx, args.out, 1 << args.max_log2n)
+    print(json.dumps(table(ctx, 1 << args.max_log2n)), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
