@@ -97,7 +97,8 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
         constexpr uint32_t G = BRT_LONG_GROUP;
         for (uint32_t q = threadIdx.x / G; q < nlong; q += TREE_BLOCK / G) {
             BrtResult r;
-            brt_node_group<K, G>(view, (uint32_t)nk, s_long[q], ec != nullptr, &r);
+            // Every probe up to BRT_LIMIT held in the window search.
+            brt_node_group<K, G>(view, (uint32_t)nk, s_long[q], ec != nullptr, &r, (uint32_t)__builtin_ctz(BRT_LIMIT) + 1u);
             if (threadIdx.x % G == 0) brt_store(out, s_long[q], r);
         }
     }

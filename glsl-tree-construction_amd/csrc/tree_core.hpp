@@ -293,7 +293,7 @@ TC_DEV bool brt_node(View codes, uint32_t nk, uint32_t i, bool want_ec, BrtResul
 constexpr int BRT_LONG_GROUP = 8;
 
 template <typename K, int G, typename View = CodeView<K>>
-TC_DEV void brt_node_group(View codes, uint32_t nk, uint32_t i, bool want_ec, BrtResult* out) {
+TC_DEV void brt_node_group(View codes, uint32_t nk, uint32_t i, bool want_ec, BrtResult* out, uint32_t e_start = 1) {
     static_assert(G >= 2 && G <= 32 && (64 % G) == 0, "groups tile the wave");
     const uint32_t lane = lane_id();
     const uint32_t gl = lane % G;         // lane within the group
@@ -305,8 +305,10 @@ TC_DEV void brt_node_group(View codes, uint32_t nk, uint32_t i, bool want_ec, Br
     auto at = [&](uint32_t x) { return fwd ? i + x : i - x; };
     const int dmin = delta(fwd ? i - 1 : i + 1);
     // lmax = 2^e for the first e >= 1 with delta(at(2^e)) <= dmin (e <= 31:
-    // 2^31 > nk, outside the keys).
-    uint32_t e0 = 1, efail;
+    // 2^31 > nk, outside the keys).  A caller that has already seen the probes
+    // up to 2^(e_start - 1) hold starts there (one doubling round less for the
+    // ranges that round would have spent below the window's bound).
+    uint32_t e0 = e_start, efail;
     while (true) {
         const uint32_t e = e0 + gl;
         const uint32_t m = group_bits(e <= 31 && delta(at(1u << e)) > dmin);
@@ -328,7 +330,9 @@ TC_DEV void brt_node_group(View codes, uint32_t nk, uint32_t i, bool want_ec, Br
         }
         return lo;
     };
-    const uint32_t l = search(0, 1u << efail, dmin);
+    // The probe at 2^(efail - 1) holds (with distinct codes the one at 1 always
+    // does), so l lies in [2^(efail - 1), 2^efail).
+    const uint32_t l = search(1u << (efail - 1), 1u << efail, dmin);
     const uint32_t j = at(l);
     const int dnode = delta(j);
     const uint32_t s = search(0, l, dnode);
