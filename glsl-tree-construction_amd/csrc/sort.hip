@@ -217,11 +217,15 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_hist_finish(uint32_t* __restrict
     if (b0 < bins) {
         for (uint32_t j = 0; j < bpt; ++j) {
             uint32_t v = 0;
-#pragma unroll 8
+            // All the copies' loads issued together (one round trip; eight at a
+            // time took four: the build 0.9888 -> 0.9853 ms).
+            uint32_t part[HIST_COPIES];
+#pragma unroll
+            for (int r = 0; r < HIST_COPIES; ++r) part[r] = acc[r * acc_stride + p * HSTRIDE + b0 + j];
+#pragma unroll
             for (int r = 0; r < HIST_COPIES; ++r) {
-                uint32_t* q = &acc[r * acc_stride + p * HSTRIDE + b0 + j];
-                v += *q;
-                *q = 0u;
+                v += part[r];
+                acc[r * acc_stride + p * HSTRIDE + b0 + j] = 0u;
             }
             c[j] = v;
             sum += v;
@@ -265,12 +269,24 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_global_histogram(
         typename KeyVec<K>::V v[HG_ITEMS];
 #pragma unroll
         for (int u = 0; u < HG_ITEMS; ++u) v[u] = ld_stream(&kv[c * CHUNK + u * HIST_BLOCK + tid]);
+        K k[HG_ITEMS][VN];
 #pragma unroll
-        for (int u = 0; u < HG_ITEMS; ++u) {
-            K k[VN];
-            vec_get<K>(v[u], k);
+        for (int u = 0; u < HG_ITEMS; ++u) vec_get<K>(v[u], k[u]);
+        // The wave-uniform test only when the chunk's first top digit is
+        // shared by the wave (as in k_morton_hist): on keys in no particular
+        // order it never pays (2^26 random u32 keys: 85.5 -> 81.5 us with the
+        // grid below).
+        const uint32_t top = hc.top_digit(k[0][0], pl);
+        if (__ballot(top == __builtin_amdgcn_readfirstlane(top)) == __ballot(true)) {
 #pragma unroll
-            for (int j = 0; j < VN; ++j) hc.add(k[j], pl);
+            for (int u = 0; u < HG_ITEMS; ++u)
+#pragma unroll
+                for (int j = 0; j < VN; ++j) hc.add(k[u][j], pl);
+        } else {
+#pragma unroll
+            for (int u = 0; u < HG_ITEMS; ++u)
+#pragma unroll
+                for (int j = 0; j < VN; ++j) hc.add_plain(k[u][j], pl);
         }
     }
     for (uint32_t v = full * CHUNK + gtid; v < nvec; v += gstride) {
@@ -482,15 +498,15 @@ int dispatch_pass(tc_ctx* ctx, SortShape sh, uint32_t tiles, hipStream_t s, cons
     return TC_ERR_INVALID;
 }
 
-// Several times as many workgroups as are resident at once (32 per CU for u32
-// keys, 20 for u64 keys): each streams a few chunks and the hardware hands the
-// next workgroup to whichever CU frees up first.  Measured on the fused Morton + histogram
-// kernel at 2^26 points: 0.248 ms with 8 workgroups per CU, 0.240 with 16,
-// 0.233 with 32.  The merge's atomic chains stay gridDim / HIST_COPIES long.
+// Histogram grids: 8 workgroups per CU for u32 keys (2^26 keys: 88.5 -> 86.7 us
+// on the headline's Morton codes, 85.5 -> 81.5 us on random keys, with the
+// per-chunk uniform test; 32 per CU before, when every key took the test) and
+// 20 for u64 keys (their 32 KiB LDS tables fit 5 per CU, so four times the
+// resident count).  The merge's atomic chains stay gridDim / HIST_COPIES long.
 template <typename K>
 uint32_t hist_grid(tc_ctx* ctx, uint32_t n) {
     uint32_t blocks = ceil_div_u32(n, HIST_BLOCK * 16);
-    uint32_t cap = (uint32_t)ctx->num_cus * (sizeof(K) == 4 ? 32u : 20u);
+    uint32_t cap = (uint32_t)ctx->num_cus * (sizeof(K) == 4 ? 8u : 20u);
     if (blocks < 1) blocks = 1;
     return blocks < cap ? blocks : cap;
 }
