@@ -79,7 +79,7 @@ constexpr int SEG_ITEMS = 8;  // point loads in flight per lane in the Morton ke
 constexpr int SEG_BRT_TILE = 1024;  // slots per workgroup step of the tree kernels
 constexpr int SEG_OCT_HALO = 128;   // LDS window halo of the octree walk
 constexpr int SEG_SEARCH_HALO = 128;     // LDS window halo of the radix-tree searches
-constexpr uint32_t SEG_BRT_LIMIT = 64;  // radix ranges searched in LDS only (within SEG_SEARCH_HALO)
+constexpr uint32_t SEG_BRT_LIMIT = 128;  // radix ranges searched in LDS only (within SEG_SEARCH_HALO; 64 before: 1,024-cloud replay 5.11 -> 5.05 ms)
 
 // Plan views.
 constexpr uint32_t PLAN_IN_PTRS = 1u;   // desc[3]: cloud c is read from in_ptr[c]

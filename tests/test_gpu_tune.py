@@ -44,3 +44,27 @@ def test_autotune_table_and_sorts(oracle, tclib, tmp_path):
         assert tc.tune.shape(ctx, 32, False, 1 << 16) == (1024, 16, 11)
     finally:
         ctx.close()
+
+
+def test_tuner_command_line_writes_a_loadable_table(tclib, tmp_path):
+    """`python -m tc.tune --out PATH --max-log2n N`: the table it writes holds
+    built shapes for every variant and size bucket up to 2^N and loads back."""
+    import json
+
+    import tc
+
+    path = str(tmp_path / "cli.json")
+    assert tc.tune.main(["--out", path, "--max-log2n", "14"]) == 0
+    with open(path) as f:
+        t = json.load(f)
+    shapes = set(tc.tune.built_shapes())
+    assert set(t) == {"u32_keys", "u32_pairs", "u64_keys", "u64_pairs"}
+    for entries in t.values():
+        assert set(entries) == {"12", "14"}
+        assert all(tuple(v) in shapes for v in entries.values())
+    ctx = tc.Context(0)
+    try:
+        tc.tune.load(ctx, path)
+        assert {k: {str(lg): v for lg, v in e.items()} for k, e in tc.tune.table(ctx, 1 << 14).items()} == t
+    finally:
+        ctx.close()
