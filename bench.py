@@ -14,6 +14,9 @@ Also reported (same JSON line):
                 its own stream (outside the timed loop)
   indexed_build the headline build with the point -> leaf map (point index
                 sorted along with the codes, run starts from unique)
+  pcie_inclusive  the headline build for points and results in host memory:
+                points copied in from pinned memory, unique codes and octree
+                nodes copied back (the cost of the host boundary; not `value`)
   sort          standalone u32 key sort of the same codes (histogram + 4
                 passes) against the 36 B/key algorithmic traffic
   sort_pairs_u64  2^27 63-bit keys with u32 payloads, 8 passes (200 B/pair)
@@ -193,6 +196,57 @@ def run_cpu_baseline(pts, reps=5):
         "single_thread_s": round(t_st, 3),
         "single_thread_method": f"whole oracle build, median of {reps} after one warm-up",
         "std_sort_ms": round(sort_ms, 1),
+    }
+
+
+def bench_pcie(ctx, pts_h, steps=5):
+    """The headline build for a caller whose points and results live in host
+    memory: points copied in from pinned host memory, the build, the counts
+    read, then the unique codes and octree nodes copied back into pinned host
+    memory (the host-array boundary's cost; never the headline value)."""
+    import ctypes as C
+
+    import torch
+
+    import tc
+    from tc.api import _hip
+
+    n = len(pts_h)
+    host_pts = torch.from_numpy(pts_h).pin_memory()
+    dev_pts = torch.empty(host_pts.shape, dtype=host_pts.dtype, device="cuda")
+    tree = tc.Tree(ctx, n, 30)
+    dev_pts.copy_(host_pts)
+    tree.build(dev_pts, n, 0.0, 1024.0)
+    cnt = tree.read_counts()
+    host_u = torch.empty(max(n, 1), dtype=torch.int32).pin_memory()
+    host_nodes = torch.empty(max(cnt["n_nodes"], 1) * 16, dtype=torch.int32).pin_memory()
+    hip = _hip()
+
+    def step():
+        dev_pts.copy_(host_pts, non_blocking=True)
+        tree.build(dev_pts, n, 0.0, 1024.0)
+        c = tree.read_counts()
+        for dst, src, nbytes in ((host_u, tree.t.unique, 4 * c["n_unique"]), (host_nodes, tree.t.nodes, 64 * c["n_nodes"])):
+            if nbytes and hip.hipMemcpy(C.c_void_p(dst.data_ptr()), C.c_void_p(src), C.c_size_t(nbytes), 2) != 0:
+                raise RuntimeError("hipMemcpy")
+        return c
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c = step()
+    torch.cuda.synchronize()
+    s = (time.perf_counter() - t0) / steps
+    tree.free()
+    moved = 16 * n + 4 * c["n_unique"] + 64 * c["n_nodes"]
+    return {
+        "workload": "the headline build with points in pinned host memory: copy in, build, unique codes and octree "
+                    "nodes copied back to pinned host memory",
+        "ms": round(s * 1e3, 3),
+        "Mpoints_per_s": round(n / s / 1e6, 1),
+        "pcie_bytes": moved,
+        "steps": steps,
     }
 
 
@@ -534,6 +588,11 @@ def main():
     }
     tree.free()
     if not args.no_extras:
+        # The standalone sort of the same codes first, as the build's own passes are timed.
+        codes = torch.empty(n, dtype=torch.int32, device="cuda")
+        ctx.morton30(pts, codes, 0.0, 1024.0)
+        out["sort"] = bench_sort_u32(ctx, codes, 20)
+        del codes
         # Same build with the point -> leaf map (index payload through the sort,
         # run starts from unique).
         itree = tc.Tree(ctx, n, 30, point_index=True)
@@ -545,10 +604,7 @@ def main():
             "ms": round(ims, 4),
             "Mpoints_per_s": round(n / (ims * 1e-3) / 1e6, 2),
         }
-        codes = torch.empty(n, dtype=torch.int32, device="cuda")
-        ctx.morton30(pts, codes, 0.0, 1024.0)
-        out["sort"] = bench_sort_u32(ctx, codes, 20)
-        del codes
+        out["pcie_inclusive"] = bench_pcie(ctx, pts_h)
         del pts
         torch.cuda.empty_cache()
         out["sort_pairs_u64"] = bench_sort_pairs_u64(ctx, 27, 2)
