@@ -83,7 +83,8 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_build_radix_tree(
             idx[k] = (uint32_t)(i < nb ? i : t0);
         }
         BrtResult r[NPL];
-        const uint32_t near = brt_nodes_near<K, BRT_LIMIT, NPL>(as_lds<K>(s_win), (int32_t)ws, idx, ec != nullptr, r);
+        const uint32_t near =
+            brt_nodes_near<K, BRT_LIMIT, NPL, sizeof(K) == 4>(as_lds<K>(s_win), (int32_t)ws, idx, ec != nullptr, r);
 #pragma unroll
         for (int k = 0; k < NPL; ++k) {
             const int64_t i = t0 + k * TREE_BLOCK + threadIdx.x;

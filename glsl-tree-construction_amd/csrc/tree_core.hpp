@@ -373,7 +373,7 @@ TC_DEV void brt_node_group(View codes, uint32_t nk, uint32_t i, bool want_ec, Br
 // dependent reads, and the NPL chains overlap.  Returns bit n set for node n
 // resolved here; the others reach LIMIT keys (or are not nodes: the caller
 // masks them) and take the full search.
-template <typename K, uint32_t LIMIT, int NPL>
+template <typename K, uint32_t LIMIT, int NPL, bool UNBOUNDED = false>
 TC_DEV uint32_t brt_nodes_near(lds_ptr<K> w, int32_t w0, const uint32_t (&idx)[NPL], bool want_ec,
                                BrtResult (&res)[NPL]) {
     static_assert(LIMIT >= 2 && (LIMIT & (LIMIT - 1)) == 0, "LIMIT is a power of two");
@@ -390,6 +390,21 @@ TC_DEV uint32_t brt_nodes_near(lds_ptr<K> w, int32_t w0, const uint32_t (&idx)[N
     uint32_t lmax[NPL], l[NPL], s[NPL], j[NPL];
     int dnode[NPL];
     uint32_t resolved = 0;
+    // UNBOUNDED: the searches run without bound tests.  delta(i, i + d x) does not increase
+    // with x (distinct codes; window slots outside the cloud hold the
+    // sentinel), so every probe at or past the range end fails for the range
+    // search, and at or past l for the split search (delta there is at most
+    // dnode); every probe stays below LIMIT keys from the node, inside the
+    // window, and the results of unresolved nodes are discarded.  A probe's
+    // address is then one multiply-add from the node's slot and a signed byte
+    // step.  The plain kernel takes this form for u32 codes (2^26 codes:
+    // uniform 0.703 -> 0.685 ms, the headline's 0.052 -> 0.049 ms, at 85
+    // VGPRs); with u64 codes it held 124 VGPRs, four waves per SIMD, and
+    // slowed (2^24 63-bit codes 0.211 -> 0.227 ms), and the segmented kernel,
+    // held to 64 VGPRs, would spill 30 of them.
+    using lds_bytes = const __attribute__((address_space(3))) char*;
+    lds_bytes base[NPL];
+    int32_t step[NPL];
 #pragma unroll
     for (int n = 0; n < NPL; ++n) {
         const uint32_t i = idx[n];
@@ -398,11 +413,14 @@ TC_DEV uint32_t brt_nodes_near(lds_ptr<K> w, int32_t w0, const uint32_t (&idx)[N
         const int dr = dlt(ci[n], code(i + 1));
         fwd[n] = dr >= dl;
         tmin[n] = thr(fwd[n] ? dl : dr);
+        base[n] = (lds_bytes)(w + ((int32_t)i - w0));
+        step[n] = fwd[n] ? (int32_t)sizeof(K) : -(int32_t)sizeof(K);
     }
-    auto probe = [&](int n, uint32_t x, K t) {
-        const uint32_t at = fwd[n] ? idx[n] + x : idx[n] - x;
-        return (K)(ci[n] ^ code(at)) < t;
+    auto at = [&](int n, uint32_t x) -> K {
+        if constexpr (UNBOUNDED) return *(lds_ptr<K>)(base[n] + __mul24(step[n], (int32_t)x));
+        else return code(fwd[n] ? idx[n] + x : idx[n] - x);
     };
+    auto probe = [&](int n, uint32_t x, K t) { return (K)(ci[n] ^ at(n, x)) < t; };
 #pragma unroll
     for (int n = 0; n < NPL; ++n) {
         int e = LOG + 1;
@@ -413,22 +431,22 @@ TC_DEV uint32_t brt_nodes_near(lds_ptr<K> w, int32_t w0, const uint32_t (&idx)[N
         lmax[n] = e <= LOG ? 1u << e : 1u;  // unresolved nodes search nothing more here
         l[n] = 0;
     }
-    // Branch-free steps (a probe past the bound reads slot 0 of the search,
-    // the node itself, and is discarded), so the NPL reads of a step issue
-    // together instead of each waiting behind a branch.
+    // Branch-free steps (bounded form: a probe past the bound reads slot 0 of
+    // the search, the node itself, and is discarded), so the NPL reads of a
+    // step issue together instead of each waiting behind a branch.
 #pragma unroll
     for (uint32_t t = LIMIT / 2; t >= 1; t /= 2)
 #pragma unroll
         for (int n = 0; n < NPL; ++n) {
             const uint32_t x = l[n] + t;
-            const bool in = x < lmax[n];
+            const bool in = UNBOUNDED || x < lmax[n];
             const bool p = probe(n, in ? x : 0u, tmin[n]);
             l[n] = (in && p) ? x : l[n];
         }
 #pragma unroll
     for (int n = 0; n < NPL; ++n) {
         j[n] = fwd[n] ? idx[n] + l[n] : idx[n] - l[n];
-        dnode[n] = dlt(ci[n], code(j[n]));
+        dnode[n] = dlt(ci[n], at(n, l[n]));
         tnode[n] = thr(dnode[n]);
         s[n] = 0;
     }
@@ -437,7 +455,7 @@ TC_DEV uint32_t brt_nodes_near(lds_ptr<K> w, int32_t w0, const uint32_t (&idx)[N
 #pragma unroll
         for (int n = 0; n < NPL; ++n) {
             const uint32_t x = s[n] + t;
-            const bool in = x < l[n];
+            const bool in = UNBOUNDED || x < l[n];
             const bool p = probe(n, in ? x : 0u, tnode[n]);
             s[n] = (in && p) ? x : s[n];
         }
