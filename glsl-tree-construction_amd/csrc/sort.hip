@@ -501,8 +501,8 @@ int dispatch_pass(tc_ctx* ctx, SortShape sh, uint32_t tiles, hipStream_t s, cons
 // Histogram grids: 8 workgroups per CU for u32 keys (2^26 keys: 88.5 -> 86.7 us
 // on the headline's Morton codes, 85.5 -> 81.5 us on random keys, with the
 // per-chunk uniform test; 32 per CU before, when every key took the test) and
-// 20 for u64 keys (their 32 KiB LDS tables fit 5 per CU, so four times the
-// resident count).  The merge's atomic chains stay gridDim / HIST_COPIES long.
+// 20 for u64 keys (unchanged).  The merge's atomic chains stay gridDim /
+// HIST_COPIES long.
 template <typename K>
 uint32_t hist_grid(tc_ctx* ctx, uint32_t n) {
     uint32_t blocks = ceil_div_u32(n, HIST_BLOCK * 16);

@@ -397,9 +397,9 @@ TC_DEV uint32_t brt_nodes_near(lds_ptr<K> w, int32_t w0, const uint32_t (&idx)[N
     // dnode); every probe stays below LIMIT keys from the node, inside the
     // window, and the results of unresolved nodes are discarded.  A probe's
     // address is then one multiply-add from the node's slot and a signed byte
-    // step.  The plain kernel takes this form for u32 codes (2^26 codes:
-    // uniform 0.703 -> 0.685 ms, the headline's 0.052 -> 0.049 ms, at 85
-    // VGPRs); with u64 codes it held 124 VGPRs, four waves per SIMD, and
+    // step.  The plain kernel takes this form for u32 codes (uniform 2^26
+    // codes: 0.694 -> 0.679 ms, at 85 VGPRs, six waves per SIMD instead of
+    // eight); with u64 codes it held 124 VGPRs, four waves per SIMD, and
     // slowed (2^24 63-bit codes 0.211 -> 0.227 ms), and the segmented kernel,
     // held to 64 VGPRs, would spill 30 of them.
     using lds_bytes = const __attribute__((address_space(3))) char*;
