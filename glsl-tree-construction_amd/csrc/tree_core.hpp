@@ -330,9 +330,12 @@ TC_DEV void brt_node_group(View codes, uint32_t nk, uint32_t i, bool want_ec, Br
         }
         return lo;
     };
-    // The probe at 2^(efail - 1) holds (with distinct codes the one at 1 always
-    // does), so l lies in [2^(efail - 1), 2^efail).
-    const uint32_t l = search(1u << (efail - 1), 1u << efail, dmin);
+    // l lies in [2^(efail - 1), 2^efail) when that lower probe is known to
+    // hold: tested here (efail > e_start) or by the caller (e_start > 1).  From
+    // e_start = 1 with efail = 1 the search starts at 0, as the probe at 1 can
+    // fail on equal codes.
+    const uint32_t lo0 = (efail > e_start || e_start > 1) ? 1u << (efail - 1) : 0u;
+    const uint32_t l = search(lo0, 1u << efail, dmin);
     const uint32_t j = at(l);
     const int dnode = delta(j);
     const uint32_t s = search(0, l, dnode);
