@@ -155,7 +155,7 @@ def host_threads():
 
 
 def run_cpu_baseline(pts, reps=5):
-    """The CPU port of the pipeline (the oracle, gcc/g++ -O3) on this host, timed
+    """The CPU port of the pipeline (the oracle, gcc/g++ -O3 -march=x86-64-v3) on this host, timed
     per stage: one warm-up, then the median of `reps` runs per stage; multi-
     threaded on the host's cores (the value reported), single-threaded, and
     std::sort of the codes (the reference's only CPU stage)."""
@@ -188,7 +188,8 @@ def run_cpu_baseline(pts, reps=5):
         "cores": threads,
         "kind": "port",
         "cpu": cpu_model(),
-        "sample": f"full workload ({n} points): the C port of the pipeline (the oracle's stages, gcc/g++ -O3) on "
+        "sample": f"full workload ({n} points): the C port of the pipeline (the oracle's stages, "
+                  f"gcc/g++ -O3 -march=x86-64-v3) on "
                   f"{threads} host threads; per stage one warm-up then the median of {reps} runs, summed: "
                   f"{t_mt:.3f} s per build (output arrays allocated beforehand, as for the GPU build)",
         "stage_ms": {k: round(v, 2) for k, v in stages.items()},
