@@ -89,6 +89,45 @@ def traffic_probe(ctx, pts_h):
     torch.cuda.synchronize()
     print(json.dumps({"traffic_probe": tree.read_counts(), "check": total}), flush=True)
     tree.free()
+
+
+def achievable_bandwidth(ctx, reps=20):
+    """What a plain streaming kernel reaches on this GPU, the practical ceiling
+    under the 8 TB/s peak: tc_traffic_reference's copy (non-temporal 16-byte
+    loads, 16-byte stores, one contiguous 32 KiB chunk per workgroup) of 256 MiB
+    (the keys of one binning pass at 2^26 u32 keys, in and out) and of 512 MiB,
+    and its non-temporal read of 1 GiB (the headline's points); each averaged
+    over `reps` back-to-back launches timed by the host."""
+    import ctypes as C
+
+    import torch
+
+    import tc
+
+    L = tc.lib()
+    buf = torch.ones(1 << 28, dtype=torch.int32, device="cuda")  # 1 GiB
+    sink = torch.zeros(2048, dtype=torch.int32, device="cuda")
+    out = {}
+    for name, mode, nbytes, moved in (("copy_256MiB", 3, 1 << 29, 1 << 29), ("copy_512MiB", 3, 1 << 30, 1 << 30),
+                                      ("read_1GiB", 1, 1 << 30, 1 << 30)):
+        def launch():
+            st = L.tc_traffic_reference(ctx.handle, C.c_void_p(buf.data_ptr()), nbytes, mode,
+                                        C.c_void_p(sink.data_ptr()), None)
+            assert st == 0, st
+
+        for _ in range(3):
+            launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            launch()
+        torch.cuda.synchronize()
+        out[name + "_GBps"] = round(moved / ((time.perf_counter() - t0) / reps) / 1e9, 1)
+    del buf
+    torch.cuda.empty_cache()
+    return out
+
+
 METRIC = "Mpoints/s full tree build @2^26 (% HBM roofline); batched vs plain p99 latency"
 
 
@@ -578,6 +617,7 @@ def main():
             "alg_bytes_per_launch": d["alg_bytes"],
             "avg_launch_ms": d["avg_ms"],
         },
+        "achievable": None,
         "build_roofline": {
             "alg_bytes": build_bytes,
             "GBps": round(build_bytes / step_s / 1e9, 1),
@@ -588,6 +628,11 @@ def main():
         "sort_tuning": tuning,
     }
     tree.free()
+    ach = achievable_bandwidth(ctx)
+    out["achievable"] = dict(ach, kernel="tc_traffic_reference: copies of 256 and 512 MiB (bytes read + written) "
+                                          "and a read of 1 GiB; non-temporal 16-byte loads, 16-byte stores",
+                             binning_frac_of_copy=round(d["GBps"] / ach["copy_256MiB_GBps"], 4),
+                             build_frac_of_copy=round(out["build_roofline"]["GBps"] / ach["copy_512MiB_GBps"], 4))
     if not args.no_extras:
         # The standalone sort of the same codes first, as the build's own passes are timed.
         codes = torch.empty(n, dtype=torch.int32, device="cuda")

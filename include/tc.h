@@ -142,11 +142,14 @@ int tc_profile_enable(tc_ctx* ctx, int on);
 int tc_set_markers(tc_ctx* ctx, int on);
 int tc_profile_collect(tc_ctx* ctx);
 /* Measurement aid (no reference counterpart): a reference kernel of known HBM
- * traffic for calibrating hardware counters (rocprofv3 --pmc).  mode 0 reads
- * `bytes` (a multiple of 16, 16-byte aligned buf) with 16-byte vector loads,
- * mode 1 the same with non-temporal loads (the form the pipeline uses for data
- * it reads once), mode 2 writes `bytes` with 16-byte stores.  The read modes
- * write one word per workgroup to sink (TC_TRAFFIC_SINK_WORDS words). */
+ * traffic for calibrating hardware counters (rocprofv3 --pmc) and for the
+ * achievable bandwidth.  mode 0 reads `bytes` (a multiple of 16, 16-byte
+ * aligned buf) with 16-byte vector loads, mode 1 the same with non-temporal
+ * loads (the form the pipeline uses for data it reads once), mode 2 writes
+ * `bytes` with 16-byte stores, mode 3 copies the first half of buf into the
+ * second (bytes a multiple of 32; non-temporal loads, 16-byte stores).  The read
+ * modes fold every word into sink (TC_TRAFFIC_SINK_WORDS words) with atomic
+ * XOR, so the XOR of the sink's words, zeroed first, is the XOR of the data. */
 #define TC_TRAFFIC_SINK_WORDS 2048
 int tc_traffic_reference(tc_ctx* ctx, void* buf, size_t bytes, int mode, uint32_t* sink, tc_stream stream);
 int tc_profile_reset(tc_ctx* ctx);

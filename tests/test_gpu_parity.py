@@ -592,3 +592,30 @@ def test_wide_digit_plans_at_tile_edges(gpu_ctx, oracle, tclib, block, kpt, n):
             assert np.array_equal(got["nodes"].view(np.uint8).reshape(-1), want["nodes"])
     finally:
         gpu_ctx.set_sort_plan(*prev)
+
+
+def test_traffic_reference_modes(gpu_ctx, tclib):
+    """The measurement aid's kernels: the read modes fold every word into the
+    sink, the write mode fills, the copy mode copies the first half onto the
+    second; bad sizes are rejected."""
+    import ctypes as C
+
+    import torch
+
+    L = tclib.lib()
+    words = 1 << 20
+    buf = torch.arange(words, dtype=torch.int32, device="cuda")
+    sink = torch.zeros(2048, dtype=torch.int32, device="cuda")
+    ptr, nbytes = C.c_void_p(buf.data_ptr()), words * 4
+    for mode in (0, 1):
+        sink.zero_()
+        assert L.tc_traffic_reference(gpu_ctx.handle, ptr, nbytes, mode, C.c_void_p(sink.data_ptr()), None) == 0
+        folded = np.bitwise_xor.reduce(sink.cpu().numpy().astype(np.uint32))
+        assert folded == np.bitwise_xor.reduce(np.arange(words, dtype=np.uint32))
+    assert L.tc_traffic_reference(gpu_ctx.handle, ptr, nbytes, 3, None, None) == 0
+    h = buf.cpu().numpy()
+    assert np.array_equal(h[words // 2:], np.arange(words // 2, dtype=np.int32))
+    assert L.tc_traffic_reference(gpu_ctx.handle, ptr, nbytes, 2, None, None) == 0
+    assert np.all(buf.cpu().numpy().view(np.uint32) == 0x5A5A5A5A)
+    assert L.tc_traffic_reference(gpu_ctx.handle, ptr, nbytes - 16, 3, None, None) != 0  # copy needs 32-byte units
+    assert L.tc_traffic_reference(gpu_ctx.handle, ptr, nbytes, 4, None, None) != 0
