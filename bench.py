@@ -221,6 +221,7 @@ def run_cpu_baseline(pts, reps=5):
     t_st = float(np.median(st1))
     codes = oracle.compute_morton(pts, 0.0, 1024.0)
     sort_ms = float(np.median([oracle.std_sort_ms(codes) for _ in range(3)]))
+    configs = cpu_baseline_configs(oracle, threads)
     return {
         "value": round(n / t_mt / 1e6, 3),
         "unit": "Mpoints/s",
@@ -236,7 +237,64 @@ def run_cpu_baseline(pts, reps=5):
         "single_thread_s": round(t_st, 3),
         "single_thread_method": f"whole oracle build, median of {reps} after one warm-up",
         "std_sort_ms": round(sort_ms, 1),
+        "configs": configs,
     }
+
+
+def cpu_baseline_configs(oracle, threads):
+    """The CPU port on the other workloads' inputs (the same generators and
+    seeds as the GPU legs): the 2^20 uniform build (per stage median of 5,
+    and single-threaded), a 2^24-pair sample of the 2^27 u64 key-value sort
+    (the parallel LSD sort, median of 3, and std::stable_sort once), and the
+    1,024-cloud batch, whole clouds built single-threaded on the host threads
+    in parallel (wall time of the second of two runs)."""
+    import numpy as np
+
+    import tc
+
+    out = {}
+    pts1 = tc.gen_uniform(1 << 20, 0xC0FFEE01)
+    runs = []
+    for r in range(6):
+        st = {}
+        oracle.build_tree_mt(pts1, 0.0, 1024.0, 30, threads=threads, timings=st)
+        if r:
+            runs.append(sum(st.values()))
+    st1 = []
+    for r in range(4):
+        t0 = time.perf_counter()
+        oracle.build_tree(pts1, 0.0, 1024.0, 30)
+        if r:
+            st1.append(time.perf_counter() - t0)
+    out["1"] = {"workload": "2^20 points uniform in [0,1024)^3, 30-bit codes, full build",
+                "Mpoints_per_s": round(len(pts1) / (float(np.median(runs)) * 1e-3) / 1e6, 1),
+                "single_thread_Mpoints_per_s": round(len(pts1) / float(np.median(st1)) / 1e6, 1)}
+    n3 = 1 << 24
+    keys = oracle.compute_morton63(tc.gen_uniform(n3, 0xC0FFEE03, 2097152.0), 0.0, 2097152.0)
+    vals = np.arange(n3, dtype=np.uint32)
+    mt = float(np.median([oracle.sort_pairs_u64_mt(keys, vals, threads)[2] for _ in range(3)]))
+    std = oracle.std_stable_sort_pairs_u64(keys, vals)[2]
+    out["3"] = {"workload": "sample of the 2^27 u64-key + u32-payload sort: 2^24 63-bit Morton keys (xyz uniform in "
+                            "[0,2^21)^3) with their indices",
+                "Mpairs_per_s": round(n3 / (mt * 1e-3) / 1e6, 1),
+                "std_stable_sort_Mpairs_per_s": round(n3 / (std * 1e-3) / 1e6, 1)}
+    rng = np.random.default_rng(0xC0FFEE04)
+    sizes = np.round(np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 20), 1024))).astype(np.uint32)
+    clouds = [tc.gen_uniform(int(s), 0xC0FFEE04 + i) if i % 2 == 0 else
+              tc.gen_gaussian_mixture(int(s), 0xC0FFEE04 + i, 16, 4.0) for i, s in enumerate(sizes)]
+    from concurrent.futures import ThreadPoolExecutor
+
+    wall = 0.0
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        for r in range(2):
+            t0 = time.perf_counter()
+            list(ex.map(lambda c: oracle.build_tree_mt(c, 0.0, 1024.0, 30, threads=1), clouds))
+            wall = time.perf_counter() - t0
+    out["4"] = {"workload": "the 1,024-cloud batch (sizes log-uniform in [2^10, 2^20], alternating uniform and "
+                            "16-cluster sigma-4 clouds): whole clouds built single-threaded on the host threads "
+                            "in parallel, wall time of the second of two runs",
+                "Mpoints_per_s": round(float(sizes.sum()) / wall / 1e6, 1)}
+    return out
 
 
 def bench_pcie(ctx, pts_h, steps=5):
@@ -314,6 +372,23 @@ def bench_sort_u32(ctx, codes, steps):
         "GBps": round(36 * n / (ms * 1e-3) / 1e9, 1),
         "frac": round(36 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+
+
+def bench_config1(ctx, steps=200):
+    """The CPU-runnable workload on the GPU: 2^20 points uniform in
+    [0,1024)^3, 30-bit codes, the whole build (HBM-resident input)."""
+    import torch
+
+    import tc
+
+    n = 1 << 20
+    pts = torch.from_numpy(tc.gen_uniform(n, 0xC0FFEE01)).cuda()
+    tree = tc.Tree(ctx, n, 30)
+    s = timed(lambda: tree.build(pts, n, 0.0, 1024.0), steps, 20)
+    c = tree.read_counts()
+    tree.free()
+    return {"workload": "2^20 points uniform in [0,1024)^3, 30-bit codes, full build", "ms": round(s * 1e3, 4),
+            "Mpoints_per_s": round(n / s / 1e6, 1), "unique_codes": c["n_unique"], "octree_nodes": c["n_nodes"]}
 
 
 def bench_sort_pairs_u64(ctx, log2n, steps):
@@ -651,6 +726,7 @@ def main():
             "Mpoints_per_s": round(n / (ims * 1e-3) / 1e6, 2),
         }
         out["pcie_inclusive"] = bench_pcie(ctx, pts_h)
+        out["config1"] = bench_config1(ctx)
         del pts
         torch.cuda.empty_cache()
         out["sort_pairs_u64"] = bench_sort_pairs_u64(ctx, 27, 2)

@@ -66,6 +66,44 @@ void lsd_sort(K* keys, K* tmp, size_t n, int threads) {
     }
 }
 
+// The same with a u32 value carried beside each key (the key-value sort of
+// the sort-dominated workload, 2^27 u64 keys with u32 payloads).
+template <typename K>
+void lsd_sort_pairs(K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, size_t n, int threads) {
+    constexpr int DIG = (int)sizeof(K);
+    std::vector<uint32_t> cnt((size_t)threads * 256);
+    K* src = keys;
+    K* dst = ktmp;
+    uint32_t* vsrc = vals;
+    uint32_t* vdst = vtmp;
+    for (int p = 0; p < DIG; ++p) {
+        const int sh = 8 * p;
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        par(threads, n, [&](int t, size_t lo, size_t hi) {
+            uint32_t* c = &cnt[(size_t)t * 256];
+            for (size_t i = lo; i < hi; ++i) ++c[(src[i] >> sh) & 255u];
+        });
+        const int used = n < 4096 ? 1 : threads;
+        uint32_t run = 0;
+        for (int d = 0; d < 256; ++d)
+            for (int t = 0; t < used; ++t) {
+                const uint32_t c = cnt[(size_t)t * 256 + d];
+                cnt[(size_t)t * 256 + d] = run;
+                run += c;
+            }
+        par(threads, n, [&](int t, size_t lo, size_t hi) {
+            uint32_t* o = &cnt[(size_t)t * 256];
+            for (size_t i = lo; i < hi; ++i) {
+                const uint32_t j = o[(src[i] >> sh) & 255u]++;
+                dst[j] = src[i];
+                vdst[j] = vsrc[i];
+            }
+        });
+        std::swap(src, dst);
+        std::swap(vsrc, vdst);
+    }
+}
+
 template <typename K>
 uint32_t par_unique(const K* s, K* out, size_t n, int threads) {
     if (n == 0) return 0;
@@ -201,6 +239,31 @@ double tco_std_sort_ms(const uint32_t* codes, uint32_t n, uint32_t* work) {
     const auto t0 = std::chrono::steady_clock::now();
     std::sort(work, work + n);
     return ms_since(t0);
+}
+
+// Stable LSD radix sort of n u64 keys with u32 values on `threads` threads, in
+// place (tmp buffers allocated here, outside the timing); returns its ms.
+double tco_mt_sort_pairs_u64_ms(uint64_t* keys, uint32_t* vals, uint32_t n, int threads) {
+    std::vector<uint64_t> kt(n);
+    std::vector<uint32_t> vt(n);
+    const auto t0 = std::chrono::steady_clock::now();
+    lsd_sort_pairs<uint64_t>(keys, vals, kt.data(), vt.data(), n, threads < 1 ? 1 : threads);
+    return ms_since(t0);
+}
+
+// std::stable_sort of n (u64 key, u32 value) pairs by key, written back in
+// place; returns the milliseconds of the sort itself.
+double tco_std_stable_sort_pairs_u64_ms(uint64_t* keys, uint32_t* vals, uint32_t n) {
+    std::vector<std::pair<uint64_t, uint32_t>> kv(n);
+    for (uint32_t i = 0; i < n; ++i) kv[i] = {keys[i], vals[i]};
+    const auto t0 = std::chrono::steady_clock::now();
+    std::stable_sort(kv.begin(), kv.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    const double ms = ms_since(t0);
+    for (uint32_t i = 0; i < n; ++i) {
+        keys[i] = kv[i].first;
+        vals[i] = kv[i].second;
+    }
+    return ms;
 }
 
 }  // extern "C"

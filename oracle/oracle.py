@@ -73,6 +73,8 @@ def lib():
             "tco_mt_build_tree_u32": (u32, [P, u32, f32, f32, i32] + [P] * 13 + [u64, i32, P]),
             "tco_mt_build_tree_u64": (u32, [P, u32, f32, f32, i32] + [P] * 13 + [u64, i32, P]),
             "tco_std_sort_ms": (C.c_double, [P, u32, P]),
+            "tco_mt_sort_pairs_u64_ms": (C.c_double, [P, P, u32, i32]),
+            "tco_std_stable_sort_pairs_u64_ms": (C.c_double, [P, P, u32]),
         }
         for name, (res, argt) in sig.items():
             fn = getattr(L, name)
@@ -286,3 +288,21 @@ def std_sort_ms(codes: np.ndarray) -> float:
     c = np.ascontiguousarray(codes, dtype=np.uint32)
     work = np.empty_like(c)
     return float(lib().tco_std_sort_ms(_p(c), len(c), _p(work)))
+
+
+def sort_pairs_u64_mt(keys: np.ndarray, vals: np.ndarray, threads: int = 0):
+    """Stable LSD radix sort of u64 keys with u32 values on `threads` host
+    threads (0 = all cores); returns (keys, vals, milliseconds)."""
+    k = np.array(keys, dtype=np.uint64, copy=True)
+    v = np.array(vals, dtype=np.uint32, copy=True)
+    ms = lib().tco_mt_sort_pairs_u64_ms(_p(k), _p(v), len(k), threads or (os.cpu_count() or 1))
+    return k, v, float(ms)
+
+
+def std_stable_sort_pairs_u64(keys: np.ndarray, vals: np.ndarray):
+    """std::stable_sort of (key, value) pairs by key; returns (keys, vals,
+    milliseconds)."""
+    k = np.array(keys, dtype=np.uint64, copy=True)
+    v = np.array(vals, dtype=np.uint32, copy=True)
+    ms = lib().tco_std_stable_sort_pairs_u64_ms(_p(k), _p(v), len(k))
+    return k, v, float(ms)

@@ -131,6 +131,11 @@ uint32_t tco_mt_build_tree_u64(const float* pts_xyzw, uint32_t n, float min_coor
                                int link_leaves, double* stage_ms);
 /* std::sort of a copy of n codes into work; returns its milliseconds. */
 double tco_std_sort_ms(const uint32_t* codes, uint32_t n, uint32_t* work);
+/* Key-value sorts of n u64 keys with u32 values, in place, returning their
+ * milliseconds: the stable LSD radix sort on `threads` threads, and
+ * std::stable_sort of the pairs. */
+double tco_mt_sort_pairs_u64_ms(uint64_t* keys, uint32_t* vals, uint32_t n, int threads);
+double tco_std_stable_sort_pairs_u64_ms(uint64_t* keys, uint32_t* vals, uint32_t n);
 
 /* Upper bound on the octree node count for n_keys unique codes of code_bits
  * bits (1 + sum over levels of min(8^L, n_keys/2)). */

@@ -498,3 +498,23 @@ def test_mt_baseline_matches_oracle(oracle, tclib, bits, threads, n):
 def test_std_sort_baseline(oracle):
     codes = rng(3).integers(0, 1 << 30, 100_000, dtype=np.uint64).astype(np.uint32)
     assert oracle.std_sort_ms(codes) >= 0.0
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("n", [0, 1, 5000, 70_000])
+def test_pairs_sort_baselines_are_stable_sorts(oracle, threads, n):
+    """The key-value CPU baselines (the sort-dominated workload's) give numpy's
+    stable argsort order: keys with many duplicates, payload = original index."""
+    r = rng(7 + n)
+    keys = r.integers(0, 1 << 63, n, dtype=np.uint64)
+    if n:
+        keys[r.integers(0, n, n // 2)] = keys[r.integers(0, n, n // 2)]  # duplicates
+        keys[: n // 10] = 0
+    vals = np.arange(n, dtype=np.uint32)
+    order = np.argsort(keys, kind="stable")
+    k, v, ms = oracle.sort_pairs_u64_mt(keys, vals, threads=threads)
+    assert ms >= 0.0
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+    k, v, ms = oracle.std_stable_sort_pairs_u64(keys, vals)
+    assert ms >= 0.0
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
