@@ -17,9 +17,12 @@
 // (also under graph replay).  Wave 0 looks back over 64 predecessors per poll
 // (measured: reading 2-4 words per lane, or spreading the poll over 2-8 waves,
 // halves the polls but makes each slower, a net loss).  The wait is bounded: a
-// workgroup that has polled spin_limit times without an answer reduces the
-// elements before its tile itself (exact, slower), so progress never depends
-// on the order in which workgroups start.
+// look-back that has polled spin_limit times without progress stops at the
+// predecessor that has not published, the workgroup reduces that tile's
+// elements itself and the look-back goes on below it.  So progress never
+// depends on the order in which workgroups start, and a predecessor that has
+// not started (its XCD may be held by another kernel's workgroups) costs one
+// tile read, not a wait on that other kernel.
 //
 // Unique (k_unique_tiles): 512 threads x 8 rows of 16-byte vectors (16384 u32
 // or 8192 u64 keys per tile) loaded straight into registers; the key before
@@ -94,45 +97,48 @@ TC_DEV uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Look-back by wave 0; returns the exclusive prefix of `tile`, or sets
-// *gave_up after spin_limit empty polls.
-TC_DEV uint32_t wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch, uint32_t spin_limit,
-                              bool* gave_up) {
+// Look-back by wave 0 from predecessor `top` down, adding into `excl` (both
+// uniform over the wave).  Returns true once the prefix is complete (an
+// inclusive predecessor, or the start of the array); false after spin_limit
+// polls without progress: the predecessors above `top` are then added and
+// tile `top` is the one that has not published.
+TC_DEV bool wave_lookback(const uint64_t* status, uint32_t epoch, uint32_t spin_limit, int64_t& top,
+                          uint32_t& excl) {
     const uint32_t lane = lane_id();
-    uint32_t excl = 0, spins = 0;
-    int64_t top = (int64_t)tile - 1;
-    *gave_up = false;
+    uint32_t spins = 0;
     while (true) {
         const int64_t idx = top - (int64_t)lane;
         uint32_t f, v;
-        uint64_t incl_mask, upto;
-        while (true) {
-            if (idx >= 0) {
-                const uint64_t w = ld_agent64(&status[idx * STATUS_STRIDE]);
-                const uint32_t hi = (uint32_t)(w >> 32);
-                f = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
-                v = (uint32_t)w;
-            } else {
-                f = 2u;  // before tile 0: an inclusive zero
-                v = 0u;
-            }
-            incl_mask = __ballot(f == 2u);
-            const uint64_t nr = __ballot(f == 0u);
-            // Lanes up to and including the nearest inclusive predecessor.
-            upto = incl_mask ? ((incl_mask & (~incl_mask + 1)) << 1) - 1 : ~0ull;
-            if ((nr & upto) == 0) break;
-            if (++spins > spin_limit) {
-                *gave_up = true;
-                return 0u;
-            }
-            __builtin_amdgcn_s_sleep(1);
+        if (idx >= 0) {
+            const uint64_t w = ld_agent64(&status[idx * STATUS_STRIDE]);
+            const uint32_t hi = (uint32_t)(w >> 32);
+            f = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
+            v = (uint32_t)w;
+        } else {
+            f = 2u;  // before tile 0: an inclusive zero
+            v = 0u;
         }
-        const bool take = (upto >> lane) & 1ull;
-        excl += wave_sum(take ? v : 0u);
-        if (incl_mask) break;
-        top -= 64;
+        const uint64_t incl_mask = __ballot(f == 2u);
+        const uint64_t nr = __ballot(f == 0u);
+        // Lanes up to and including the nearest inclusive predecessor.
+        const uint64_t upto = incl_mask ? ((incl_mask & (~incl_mask + 1)) << 1) - 1 : ~0ull;
+        const uint64_t wait = nr & upto;
+        if (wait == 0) {
+            excl += wave_sum(((upto >> lane) & 1ull) ? v : 0u);
+            if (incl_mask) return true;
+            top -= 64;
+            continue;
+        }
+        if (++spins > spin_limit) {
+            // The lanes below the first unpublished one hold aggregates only
+            // (an inclusive one among them would have ended the walk).
+            const uint64_t ready = (wait & (~wait + 1)) - 1;
+            excl += wave_sum(((ready >> lane) & 1ull) ? v : 0u);
+            top -= (int64_t)__popcll(ready);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
-    return excl;
 }
 
 // Largest c in [0, nseg-1] with off[c] <= i (skips empty segments).
@@ -172,22 +178,62 @@ TC_DEV uint32_t elem_value(const ScanArgs& a, uint32_t n, uint32_t i, uint32_t n
     return i == 0 ? 1u : (uint32_t)(pn[i] / 3 - pn[p] / 3);
 }
 
-// The look-back's fall-back: the sum of the scan values of [0, base), by the
+// The look-back's fall-back: the sum of the scan values of [lo, hi), by the
 // whole block (each thread sums a contiguous chunk).  Every thread calls it.
 template <int MODE>
-TC_DEV uint32_t reduce_before(const ScanArgs& a, uint32_t n, uint32_t nseg, uint32_t base, uint32_t* s_tmp) {
-    const uint32_t chunk = ceil_div_u32(base, SCAN_BLOCK);
-    const uint32_t lo = threadIdx.x * chunk, hi = lo + chunk < base ? lo + chunk : base;
+TC_DEV uint32_t reduce_range(const ScanArgs& a, uint32_t n, uint32_t nseg, uint32_t lo, uint32_t hi,
+                             uint32_t* s_tmp) {
+    const uint32_t chunk = ceil_div_u32(hi - lo, SCAN_BLOCK);
+    const uint32_t t0 = lo + threadIdx.x * chunk;
+    const uint32_t t1 = t0 + chunk < hi ? t0 + chunk : hi;
     uint32_t sum = 0, seg = 0, seg_start = 0, seg_next = 0xFFFFFFFFu;
-    if (is_seg(MODE) && nseg > 0 && lo < hi) {
-        seg = find_seg(a.seg_off, nseg, lo);
+    if (is_seg(MODE) && nseg > 0 && t0 < t1) {
+        seg = find_seg(a.seg_off, nseg, t0);
         seg_start = a.seg_off[seg];
         seg_next = a.seg_off[seg + 1];
     }
-    for (uint32_t i = lo; i < hi; ++i) sum += elem_value<MODE>(a, n, i, nseg, seg, seg_start, seg_next);
+    for (uint32_t i = t0; i < t1; ++i) sum += elem_value<MODE>(a, n, i, nseg, seg, seg_start, seg_next);
     uint32_t total;
     block_exclusive_scan<SCAN_BLOCK>(sum, s_tmp, &total);
     return total;
+}
+
+// Exclusive prefix of `tile` (> 0, tiles of tile_size elements), whose
+// aggregate is published: wave 0
+// looks back; a predecessor that does not publish within spin_limit polls is
+// reduced by the whole block (spin_limit 0: every predecessor, used to test
+// that path).  Publishes the tile's inclusive value.  Every thread calls it;
+// the other waves may still be busy with their own work when they do.
+template <int MODE>
+TC_DEV uint32_t tile_prefix(const ScanArgs& a, uint32_t n, uint32_t nseg, uint32_t tile, uint32_t tile_size,
+                            uint32_t epoch, uint32_t block_total, uint32_t* s_tmp, uint32_t* s_stall,
+                            uint32_t* s_excl) {
+    const bool w0 = (threadIdx.x >> 6) == 0;
+    int64_t top = (int64_t)tile - 1;
+    uint32_t excl = 0;
+    while (true) {
+        if (w0) {
+            const bool done = top < 0 || (a.spin_limit != 0u && wave_lookback(a.status, epoch, a.spin_limit, top, excl));
+            if (lane_id() == 0) {
+                *s_stall = done ? 0u : (uint32_t)top + 1u;
+                if (done) {
+                    *s_excl = excl;
+                    st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, excl + block_total));
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t stalled = *s_stall;
+        if (stalled == 0u) break;  // uniform
+        const uint32_t lo = (stalled - 1u) * tile_size;
+        const uint32_t r = reduce_range<MODE>(a, n, nseg, lo, lo + tile_size < n ? lo + tile_size : n, s_tmp);
+        // (reduce_range's barriers order every read of *s_stall before wave 0 writes it again.)
+        if (w0) {
+            excl += r;
+            top = (int64_t)stalled - 2;
+        }
+    }
+    return *s_excl;
 }
 
 // Coalesced tile load of u32 scan values into LDS (tile-relative positions).
@@ -213,7 +259,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     static_assert(!is_unique(MODE), "unique modes run k_unique_tiles");
     __shared__ uint32_t s_tmp[SCAN_BLOCK / 64 + 2];
     __shared__ uint32_t s_excl;
-    __shared__ uint32_t s_fallback;
+    __shared__ uint32_t s_stall;  // look-back: the stalled predecessor + 1 (0: none)
     __shared__ uint32_t s_seg0;  // segmented modes: the segment of the tile's first element
     __shared__ __attribute__((aligned(16))) uint32_t s_items[SCAN_TILE];
 
@@ -291,39 +337,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(ScanArgs a) {
     uint32_t block_total;
     const uint32_t texcl = block_exclusive_scan<SCAN_BLOCK>(tsum, s_tmp, &block_total);
 
-    if (tid == 0) s_fallback = (a.spin_limit == 0u && tile > 0) ? 1u : 0u;  // 0: always reduce directly
-    __syncthreads();
-    if (wave == 0) {
-        const uint32_t lane = lane_id();
-        if (tile == 0) {
-            if (lane == 0) {
-                st_agent64(&a.status[0], st_word(2u, epoch, block_total));
-                s_excl = 0;
-            }
-        } else {
-            if (lane == 0) st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(1u, epoch, block_total));
-            if (!s_fallback) {
-                bool gave_up = false;
-                const uint32_t ex = wave_lookback(a.status, tile, epoch, a.spin_limit, &gave_up);
-                if (lane == 0) {
-                    if (gave_up) s_fallback = 1u;
-                    else s_excl = ex;
-                }
-            }
-        }
+    uint32_t tile_excl = 0;
+    if (tile == 0) {
+        if (tid == 0) st_agent64(&a.status[0], st_word(2u, epoch, block_total));
+    } else {
+        if (tid == 0) st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(1u, epoch, block_total));
+        tile_excl = tile_prefix<MODE>(a, n, nseg, tile, SCAN_TILE, epoch, block_total, s_tmp, &s_stall, &s_excl);
     }
-    __syncthreads();
-    if (s_fallback) {
-        const uint32_t before = reduce_before<MODE>(a, n, nseg, base, s_tmp);
-        if (tid == 0) {
-            s_excl = before;
-            st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, before + block_total));
-        }
-        __syncthreads();
-    } else if (wave == 0 && tile > 0 && lane_id() == 0) {
-        st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, s_excl + block_total));
-    }
-    const uint32_t tile_excl = s_excl;
 
     // Segment bases: the exclusive prefix at each segment's first element.
     if (is_seg(MODE) && start_mask) {
@@ -443,7 +463,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_unique_tiles(ScanArgs a) {
     __shared__ uint64_t s_w[NP * SCAN_WAVES];
     __shared__ uint32_t s_tmp[SCAN_WAVES + 2];
     __shared__ uint32_t s_excl;
-    __shared__ uint32_t s_fallback;
+    __shared__ uint32_t s_stall;  // look-back: the stalled predecessor + 1 (0: none)
     __shared__ uint32_t s_seg[2];  // segmented: the clouds starting in this tile, [s_seg[0], s_seg[1])
     __shared__ __attribute__((aligned(16))) T s_keys[Sh::HALF];
     // Segmented: for each thread's slots of each row, the tile-local output
@@ -549,7 +569,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_unique_tiles(ScanArgs a) {
         inc[p] = wave_inclusive_scan64(pk[p]);
         if (lane == 63) s_w[p * SCAN_WAVES + wave] = inc[p];
     }
-    if (tid == 0) s_fallback = (a.spin_limit == 0u && tile > 0) ? 1u : 0u;  // 0: always reduce directly
     __syncthreads();
     uint32_t row_base[UQ_ROWS];  // tile-local output index of each row's first kept key
     uint32_t slot[UQ_ROWS];      // ... and of this thread's first kept key in each row
@@ -608,32 +627,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_unique_tiles(ScanArgs a) {
 
     // First half into LDS while wave 0 looks back.
     compact_keys(0);
-    if (wave == 0) {
-        if (tile == 0) {
-            if (lane == 0) s_excl = 0;
-        } else if (!s_fallback) {
-            bool gave_up = false;
-            const uint32_t ex = wave_lookback(a.status, tile, epoch, a.spin_limit, &gave_up);
-            if (lane == 0) {
-                if (gave_up) {
-                    s_fallback = 1u;
-                } else {
-                    s_excl = ex;
-                    st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, ex + block_total));
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (s_fallback) {
-        const uint32_t before = reduce_before<MODE>(a, n, nseg, base, s_tmp);
-        if (tid == 0) {
-            s_excl = before;
-            st_agent64(&a.status[(size_t)tile * STATUS_STRIDE], st_word(2u, epoch, before + block_total));
-        }
-        __syncthreads();
-    }
-    const uint32_t tile_excl = s_excl;
+    uint32_t tile_excl = 0;
+    if (tile == 0) __syncthreads();  // the staged keys are read below (tile_prefix's barriers do this otherwise)
+    else tile_excl = tile_prefix<MODE>(a, n, nseg, tile, TILE, epoch, block_total, s_tmp, &s_stall, &s_excl);
 
     T* out = static_cast<T*>(a.out0);
     uint32_t* run_first = SEG ? nullptr : static_cast<uint32_t*>(a.out1);

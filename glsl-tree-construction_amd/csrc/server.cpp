@@ -219,11 +219,13 @@ void scheduler_loop(tc_server* s) {
         if (s->stop && !any_busy) break;  // requests still queued are dropped
         if (progressed) continue;
         if (any_busy) {
-            // A batch is running: poll its event again soon, but let
-            // submitters and pollers take the lock in between.
-            lk.unlock();
-            std::this_thread::yield();
-            lk.lock();
+            // A batch is running: poll its event again after a short sleep,
+            // or at once when a request or a release comes in.  The sleep
+            // gives the lock up; an unlock-yield-relock spin would win the
+            // mutex back before submitters and pollers could take it, and
+            // under load they starved for whole batches.
+            const uint64_t g = s->gen;
+            s->cv_work.wait_for(lk, std::chrono::microseconds(20), [&] { return s->gen != g; });
         } else {
             const uint64_t g = s->gen;
             s->cv_work.wait(lk, [&] { return s->gen != g; });

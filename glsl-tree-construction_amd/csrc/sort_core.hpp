@@ -114,7 +114,7 @@ struct OnesweepShared {
     uint32_t gbase[D];  // digit bases of the scatter; the fall-back's digit counts before that
     uint32_t tmp[WAVES + 2];
     uint32_t aux;
-    uint32_t fallback;
+    uint32_t fallback;  // look-back: the stalled predecessor + 1 (0: none)
 };
 
 // One digit-binning tile step (shader/new_radix_sort.comp:116-314), digit
@@ -284,65 +284,89 @@ TC_DEV void onesweep_tile_impl(OnesweepShared<K, BLOCK, KPT, RB>& sm, const K* _
     // carry the same flag), down to the segment's first tile (which always
     // publishes inclusive counts, so the walk never passes it).  Four
     // predecessors are fetched per round trip: agent-scope loads bypass the
-    // XCD-local L2, so each round trip is long.  The wait is bounded: after
-    // spin_limit empty polls the whole workgroup counts the digits of the
-    // segment's earlier keys itself (exact, just slower), so progress never
-    // depends on the order in which the hardware starts workgroups.
-    if (tid == 0) sm.fallback = (spin_limit == 0u && tile != first) ? 1u : 0u;  // 0: always count directly
+    // XCD-local L2, so each round trip is long.
+    //
+    // The wait is bounded: a walker that has polled spin_limit times without
+    // progress stops at the predecessor that has not published; the workgroup
+    // then counts that tile's digits itself (it is a full tile: only a
+    // segment's last tile is partial, and it comes after this one) and the
+    // walkers stopped there step below it and go on.  Tiles are workgroup ids,
+    // and the hardware may start a workgroup before a lower-numbered one (each
+    // XCD starts its own share in order, but an XCD held by another kernel's
+    // workgroups starts nothing): a predecessor that has not started costs one
+    // tile read here, never a wait on that other kernel.  spin_limit 0 counts
+    // every predecessor that way (used to test it).
+    // sm.fallback: the stalled predecessor + 1 (0: none).
     uint32_t excl[DPT];
 #pragma unroll
     for (int j = 0; j < DPT; ++j) excl[j] = 0;
+    bool walking = owner && tile != first;
+    uint32_t k = tile - 1;  // the next predecessor to account for
+    if (tid == 0) sm.fallback = 0u;
     __syncthreads();
-    if (owner && tile != first && spin_limit != 0u) {
-        uint32_t k = tile - 1;
-        uint32_t spins = 0;
-        while (true) {
-            uint64_t v[4];
+    while (true) {
+        if (walking && spin_limit == 0u) {
+            atomicMax(&sm.fallback, k + 1u);
+        } else if (walking) {
+            uint32_t spins = 0;
+            while (true) {
+                uint64_t v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (k >= first + q) {
-                    if constexpr (DPT == 1) v[q] = ld_agent(&lb_cur[(k - q) * ROW + d0]);
-                    else v[q] = ld_agent64(reinterpret_cast<const uint64_t*>(lb_cur + (k - q) * ROW + d0));
-                } else {
-                    v[q] = 0;
+                for (int q = 0; q < 4; ++q) {
+                    if (k >= first + q) {
+                        if constexpr (DPT == 1) v[q] = ld_agent(&lb_cur[(k - q) * ROW + d0]);
+                        else v[q] = ld_agent64(reinterpret_cast<const uint64_t*>(lb_cur + (k - q) * ROW + d0));
+                    } else {
+                        v[q] = 0;
+                    }
                 }
-            }
-            bool found = false;
-            uint32_t used = 0;
+                bool found = false;
+                uint32_t used = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (found || used != (uint32_t)q) break;
-                const uint32_t lo = (uint32_t)v[q];
-                const uint32_t f = lo & LB_FLAG_MASK;
-                if (f == LB_NOT_READY) break;
-                excl[0] += lo >> 2;
-                if constexpr (DPT == 2) excl[1] += (uint32_t)(v[q] >> 32) >> 2;
-                if (f == LB_INCLUSIVE) found = true;
-                else ++used;
-            }
-            if (found) break;
-            k -= used;
-            if (used < 4) {
-                if (++spins > spin_limit) {
-                    sm.fallback = 1u;
+                for (int q = 0; q < 4; ++q) {
+                    if (found || used != (uint32_t)q) break;
+                    const uint32_t lo = (uint32_t)v[q];
+                    const uint32_t f = lo & LB_FLAG_MASK;
+                    if (f == LB_NOT_READY) break;
+                    excl[0] += lo >> 2;
+                    if constexpr (DPT == 2) excl[1] += (uint32_t)(v[q] >> 32) >> 2;
+                    if (f == LB_INCLUSIVE) found = true;
+                    else ++used;
+                }
+                if (found) {
+                    walking = false;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
+                k -= used;
+                if (used < 4) {
+                    if (++spins > spin_limit) {
+                        atomicMax(&sm.fallback, k + 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
             }
         }
-    }
-    __syncthreads();
-    if (sm.fallback) {
+        __syncthreads();
+        const uint32_t stalled = sm.fallback;
+        if (stalled == 0u) break;  // uniform: read by every thread after the barrier
+        // Digit counts of predecessor stalled - 1, into gbase (free until the
+        // scatter bases are written below).
         uint32_t* fb = sm.gbase;
         for (uint32_t i = tid; i < D; i += BLOCK) fb[i] = 0u;
         __syncthreads();
-        for (uint32_t i = seg_base + tid; i < base; i += BLOCK) atomicAdd(&fb[digit_of<K, RB>(sk[i], shift)], 1u);
+        if (tid == 0) sm.fallback = 0u;  // every thread has read it
+        const K* tk = sk + seg_base + (size_t)(stalled - 1u - first) * Sh::TILE;
+        for (uint32_t i = tid; i < Sh::TILE; i += BLOCK) atomicAdd(&fb[digit_of<K, RB>(tk[i], shift)], 1u);
         __syncthreads();
-        if (owner) {
+        if (walking && k + 1u == stalled) {
 #pragma unroll
-            for (int j = 0; j < DPT; ++j) excl[j] = fb[d0 + j];
+            for (int j = 0; j < DPT; ++j) excl[j] += fb[d0 + j];
+            if (k == first) walking = false;
+            else --k;
         }
-        __syncthreads();  // the counts are read before gbase is written below
+        // The next round's barrier orders these reads before fb is cleared
+        // again; the final round's orders them before gbase is written.
     }
     if (owner) {
         if (tile != first) {

@@ -89,9 +89,12 @@ struct tc_ctx {
     // [key bytes 4 / 8][keys only / pairs][ceil(log2 n)], -1 where not measured.
     bool tune_on = false;
     int8_t tune[2][2][32];
-    // Look-back wait bound (empty polls) before a workgroup computes its
-    // prefix itself; tests set it to 0 to exercise that path.
-    uint32_t spin_limit = 1u << 15;
+    // Look-back wait bound: polls without progress (about 0.8 us each) before
+    // a workgroup computes the stalled predecessor's counts itself, one tile
+    // read.  Short, since a predecessor that has not started may wait for
+    // another stream's kernel to leave its XCD; tests set 0 (every
+    // predecessor computed) and 1 (most of them) to exercise that path.
+    uint32_t spin_limit = TC_LOOKBACK_SPIN_DEFAULT;
     // Grow-only device scratch.
     void* scratch = nullptr;
     size_t scratch_bytes = 0;

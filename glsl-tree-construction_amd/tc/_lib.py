@@ -15,6 +15,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libtc.so")
 
 TC_OK = 0
+LOOKBACK_SPIN_DEFAULT = 64  # TC_LOOKBACK_SPIN_DEFAULT in tc.h
 STATUS_NAMES = {
     0: "TC_OK", -1: "TC_ERR_INVALID", -2: "TC_ERR_NOMEM", -3: "TC_ERR_TOO_LARGE", -4: "TC_ERR_HIP",
     -5: "TC_ERR_OVER_BUDGET", -6: "TC_ERR_CAPACITY", -7: "TC_ERR_BUSY", -8: "TC_ERR_NOT_FOUND",

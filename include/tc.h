@@ -119,10 +119,13 @@ int tc_clear_sort_tuning(tc_ctx* ctx);
  * choices, without their digit plans; tc_sort_plan_list has those): writes up
  * to max (block, keys per thread) pairs and returns how many there are. */
 int tc_sort_shape_list(int* blocks, int* keys_per_thread, int max);
-/* Bound on a workgroup's look-back wait (empty polls of a predecessor's
- * status) in the sort and scan kernels; past it the workgroup computes its
- * prefix from the input itself, so progress never depends on workgroup start
- * order.  0 makes every workgroup take that path (used to test it). */
+/* Bound on a workgroup's look-back wait (polls of a predecessor's status
+ * without progress) in the sort and scan kernels; past it the workgroup
+ * computes that predecessor tile's counts from the input itself and looks on
+ * below it, so progress never depends on workgroup start order.  0 makes every
+ * workgroup compute every predecessor that way (used to test it).  Default
+ * TC_LOOKBACK_SPIN_DEFAULT. */
+#define TC_LOOKBACK_SPIN_DEFAULT 64u
 int tc_set_lookback_spin_limit(tc_ctx* ctx, uint32_t polls);
 
 /* Per-launch event profiler (the reference times each stage with

@@ -130,20 +130,23 @@ def test_batch_rejects_over_capacity(gpu_ctx):
     b.close()
 
 
-def test_batch_lookback_fallback_path(gpu_ctx, oracle, tclib):
+@pytest.mark.parametrize("limit", [0, 1])
+def test_batch_lookback_fallback_path(gpu_ctx, oracle, tclib, limit):
+    """Segmented sort and scans with predecessor tiles computed by the waiting
+    workgroup (all of them at 0, most at 1): same trees as the oracle's."""
     import tc
 
     L = tc.lib()
-    clouds = make_clouds(tclib, [20_000, 3, 50_000, 1, 9000], 4242)
-    assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 0) == 0
+    clouds = make_clouds(tclib, [20_000, 3, 50_000, 1, 9000, 130_000], 4242)
+    assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, limit) == 0
     try:
-        b = tc.Batch(gpu_ctx, 8, 100_000, 30)
+        b = tc.Batch(gpu_ctx, 8, 250_000, 30)
         run_batch(b, clouds)
         for i, (g, pts) in enumerate(zip(b.download(), clouds)):
             check_cloud(oracle, g, pts, 0.0, 1024.0, 30, f"cloud {i}")
         b.close()
     finally:
-        assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 1 << 15) == 0
+        assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, tc._lib.LOOKBACK_SPIN_DEFAULT) == 0
 
 
 def test_batch_clouds_at_unique_tile_edges(gpu_ctx, oracle, tclib):

@@ -412,15 +412,18 @@ def test_invalid_arguments_rejected(gpu_ctx):
     assert L.tc_sort_keys_u32(h, k.data_ptr(), k.data_ptr(), (1 << 30) + 1, None) == tc._lib.TC_ERR_TOO_LARGE
 
 
-def test_lookback_fallback_path(gpu_ctx, oracle, tclib):
-    """With the look-back wait bound at 0 every workgroup computes its prefix
-    from the input itself; results must be identical to the look-back path."""
+@pytest.mark.parametrize("limit", [0, 1])
+def test_lookback_fallback_path(gpu_ctx, oracle, tclib, limit):
+    """With the look-back wait bound at 0 every workgroup computes each
+    predecessor tile's counts from the input itself; at 1 most predecessors
+    are computed and the rest read, so the walk resumes after each computed
+    tile.  Results must be identical to the look-back path's."""
     import tc
 
     L = tc.lib()
     pts = tclib.gen_gaussian_mixture(300_000, 77, 32, 6.0)
     want = oracle_all(oracle, pts, 0.0, 1024.0, 30)
-    assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 0) == 0
+    assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, limit) == 0
     try:
         got = run_stages(gpu_ctx, pts, 0.0, 1024.0, 30)
         assert_same(got, want)
@@ -433,7 +436,7 @@ def test_lookback_fallback_path(gpu_ctx, oracle, tclib):
         sk, sv = oracle.radix_sort(k64, vals)
         assert np.array_equal(host(k, np.uint64), sk) and np.array_equal(host(v, np.uint32), sv)
     finally:
-        assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, 1 << 15) == 0
+        assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, tc._lib.LOOKBACK_SPIN_DEFAULT) == 0
 
 
 def test_reference_sort_input_10M_descending(gpu_ctx, oracle):

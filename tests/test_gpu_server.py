@@ -138,6 +138,35 @@ def test_result_same_alone_and_in_a_concurrent_mix(gpu_ctx, oracle, tclib):
         srv.close()
 
 
+def test_large_batches_overlapping_on_two_streams(gpu_ctx, oracle, tclib):
+    """Two slots whose batches of 2^19 .. 2^21-point clouds run at once: the
+    kernels of the two streams share the CUs, so a workgroup may start before
+    a lower-numbered one of its own launch, and the sort and scan look-backs
+    then compute the stalled predecessor tiles themselves.  Every result equals
+    a plain build of the same cloud (and the oracle's, for the smallest)."""
+    import tc
+
+    clouds = make_clouds(tclib, 24, seed=19, lo=2 ** 19, hi=2 ** 21)[4:]
+    srv = tc.Server(gpu_ctx, mem_budget_bytes=24 << 30, n_streams=2, max_batch_clouds=32,
+                    max_batch_points=1 << 23)
+    try:
+        d_pts = [dev(c) for c in clouds]
+        ids = [srv.submit(p, len(c)) for p, c in zip(d_pts, clouds)]
+        got = collect(srv, ids)
+        assert srv.stats()["batches"] >= 3
+        small = int(np.argmin([len(c) for c in clouds]))
+        for i, (rid, c) in enumerate(zip(ids, clouds)):
+            res = got[rid]
+            assert res["status"] == 0, i
+            tree = srv.download(res)
+            assert_same_tree(tree, plain_build(gpu_ctx, c), f"cloud {i} n={len(c)}")
+            if i == small:
+                assert_same_tree(tree, oracle_tree(oracle, c), f"cloud {i} vs oracle")
+            srv.release(rid)
+    finally:
+        srv.close()
+
+
 def test_63bit_served(gpu_ctx, oracle, tclib):
     import tc
 
