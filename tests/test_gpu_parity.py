@@ -254,6 +254,41 @@ def test_fused_pipeline_matches_oracle(gpu_ctx, oracle, tclib):
         tree.free()
 
 
+def test_builds_on_two_streams_at_once(oracle, tclib):
+    """Two contexts build on two streams at once, four builds each per round.
+    The two launches share the CUs, so a workgroup may start before a
+    lower-numbered one of its own launch. The sort and scan look-backs then
+    compute the stalled predecessor tiles themselves. Every round's trees must
+    equal the oracle's."""
+    import tc
+    import torch
+
+    clouds = [tclib.gen_gaussian_mixture(1 << 22, 91, 32, 6.0), tclib.gen_uniform(1 << 22, 92)]
+    wants = [oracle_all(oracle, c, 0.0, 1024.0, 30) for c in clouds]
+    ctxs = [tc.Context(0) for _ in clouds]
+    streams = [torch.cuda.Stream() for _ in clouds]
+    trees = [tc.Tree(x, len(c), 30) for x, c in zip(ctxs, clouds)]
+    d_pts = [dev(c) for c in clouds]
+    torch.cuda.synchronize()
+    try:
+        for rnd in range(3):
+            for _ in range(4):
+                for t, p, s, c in zip(trees, d_pts, streams, clouds):
+                    t.build(p, len(c), 0.0, 1024.0, stream=s)
+            torch.cuda.synchronize()
+            for i, (t, w) in enumerate(zip(trees, wants)):
+                got = t.download()
+                assert got["n_unique"] == len(w["unique"]), (rnd, i)
+                assert_same(got, w, ("sorted", "unique", "prefix_n", "has_leaf_left", "has_leaf_right",
+                                     "left_child", "parent", "edge_count", "offsets"))
+                assert np.array_equal(got["nodes"].view(np.uint8).reshape(-1), w["nodes"]), (rnd, i)
+    finally:
+        for t in trees:
+            t.free()
+        for x in ctxs:
+            x.close()
+
+
 def test_tree_counts_after_empty_and_refill(gpu_ctx, oracle, tclib):
     """The device counters are zeroed only for an empty cloud (unique writes
     the count otherwise): a build of 0 points after a full one reads back as
