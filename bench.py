@@ -495,6 +495,48 @@ def bench_batched(ctx, n_clouds, steps, seed=0xC0FFEE04, plain=True):
     }
 
 
+def served_pool(seed, pool):
+    """The served workload's request pool: `pool` clouds resident in HBM, sizes
+    log-uniform in [2^12, 2^22], uniform xyz in [0,1024)^3."""
+    import numpy as np
+    import torch
+
+    import tc
+
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(2 ** 12), np.log(2 ** 22), pool))).astype(np.uint32)
+    pts = [torch.from_numpy(tc.gen_uniform(int(s), seed + i)).cuda() for i, s in enumerate(sizes)]
+    return rng, sizes, pts
+
+
+def bench_served_overload(ctx, rate, n_streams, max_batch_points, requests=2000, pool=256, seed=0xC0FFEE05):
+    """The server alone on one Poisson trace of the served workload (no plain
+    leg), with a chosen stream count and batch size: the run to profile when
+    a configuration's throughput under load is in question."""
+    import numpy as np
+
+    import tc
+
+    rng, sizes, pts = served_pool(seed, pool)
+    srv = tc.Server(ctx, n_streams=n_streams, max_batch_points=max_batch_points)
+    try:
+        warm = np.arange(8) * 1e-4
+        srv.run_trace(warm, pts[:8], sizes[:8])
+        t = np.cumsum(rng.exponential(1.0 / rate, requests))
+        t -= t[0]
+        idx = rng.integers(0, pool, requests)
+        srv.reset_stats()
+        lat = srv.run_trace(t, [pts[i] for i in idx], sizes[idx])
+        st = srv.stats()
+    finally:
+        srv.close()
+    return {"rate_req_s": rate, "n_streams": n_streams, "max_batch_points": max_batch_points,
+            "requests": requests, "batches": st["batches"],
+            "Mpoints_per_s": round(float(sizes[idx].sum()) / float(np.max(t + lat)) / 1e6, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+
+
 def bench_served(ctx, rates=(1000, 2000, 5000, 7500, 10000, 12000, 14000, 16000, 18000, 20000, 50000), pool=256,
                  max_req=2000, window_s=0.5, slo_ms=10.0, seed=0xC0FFEE05):
     """Poisson arrivals of build requests (sizes log-uniform in [2^12, 2^22],
@@ -506,9 +548,7 @@ def bench_served(ctx, rates=(1000, 2000, 5000, 7500, 10000, 12000, 14000, 16000,
 
     import tc
 
-    rng = np.random.default_rng(seed)
-    sizes = np.round(np.exp(rng.uniform(np.log(2 ** 12), np.log(2 ** 22), pool))).astype(np.uint32)
-    pts = [torch.from_numpy(tc.gen_uniform(int(s), seed + i)).cuda() for i, s in enumerate(sizes)]
+    rng, sizes, pts = served_pool(seed, pool)
     srv = tc.Server(ctx)  # defaults: 64 GiB budget, 2 streams, <= 256 clouds / 2^24 points per batch
     # Warm up both paths.
     warm = np.arange(8) * 1e-4
@@ -607,6 +647,9 @@ def main():
     ap.add_argument("--retune", action="store_true", help="run the autotuner instead of loading its saved table")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched workload's graph replays (for rocprofv3 kernel statistics)")
+    ap.add_argument("--served-overload", type=int, nargs=3, metavar=("RATE", "STREAMS", "MAX_BATCH_POINTS"),
+                    help="only the server on one Poisson trace of the served workload at RATE req/s, with "
+                         "STREAMS slots and batches of at most MAX_BATCH_POINTS (for rocprofv3 kernel traces)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only build the headline tree 3 times, then run reference kernels of known traffic "
                          "(for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
@@ -636,6 +679,10 @@ def main():
         return
     if args.batched_only:
         print(json.dumps({"batched": bench_batched(ctx, 1024, args.steps, plain=False)}), flush=True)
+        return
+    if args.served_overload:
+        rate, streams, mbp = args.served_overload
+        print(json.dumps({"served_overload": bench_served_overload(ctx, rate, streams, mbp)}), flush=True)
         return
     pts = torch.from_numpy(pts_h).cuda()
     tree = tc.Tree(ctx, n, 30)
