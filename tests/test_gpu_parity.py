@@ -474,6 +474,43 @@ def test_lookback_fallback_path(gpu_ctx, oracle, tclib, limit):
         assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, tc._lib.LOOKBACK_SPIN_DEFAULT) == 0
 
 
+@pytest.mark.parametrize("limit", [0, 1])
+@pytest.mark.parametrize("block,kpt,bits", [(1024, 16, 11), (1024, 8, 11), (256, 8, 8)])
+def test_lookback_fallback_sort_plans(gpu_ctx, oracle, limit, block, kpt, bits):
+    """Sorts whose look-backs compute predecessor tiles themselves (every one
+    at limit 0, most at 1) under other plans than the tuned one: 11-bit digits
+    with two digits per look-back thread (1024-thread blocks, 2,048 digits),
+    and small 256 x 8 tiles (98 tiles, so long walks).  u32 pairs, u64 keys
+    and u64 pairs against the oracle."""
+    import tc
+
+    L = tc.lib()
+    n = 200_003
+    r = np.random.default_rng(limit * 31 + kpt + bits)
+    keys = r.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    keys[::7] = keys[3]
+    k64 = r.integers(0, 1 << 64, size=n, dtype=np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    prev = gpu_ctx.sort_plan()
+    gpu_ctx.set_sort_plan(block, kpt, bits)
+    assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, limit) == 0
+    try:
+        sk, sv = oracle.radix_sort(keys, vals)
+        k, v = dev(keys), dev(vals)
+        gpu_ctx.sort(k, empty(n, np.uint32), v, empty(n, np.uint32))
+        assert np.array_equal(host(k, np.uint32, n), sk) and np.array_equal(host(v, np.uint32, n), sv)
+        k = dev(k64)
+        gpu_ctx.sort(k, empty(n, np.uint64))
+        assert np.array_equal(host(k, np.uint64, n), np.sort(k64))
+        sk, sv = oracle.radix_sort(k64, vals)
+        k, v = dev(k64), dev(vals)
+        gpu_ctx.sort(k, empty(n, np.uint64), v, empty(n, np.uint32))
+        assert np.array_equal(host(k, np.uint64, n), sk) and np.array_equal(host(v, np.uint32, n), sv)
+    finally:
+        assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, tc._lib.LOOKBACK_SPIN_DEFAULT) == 0
+        gpu_ctx.set_sort_plan(*prev)
+
+
 def test_reference_sort_input_10M_descending(gpu_ctx, oracle):
     """The reference's own sort input, keys 10^7 ... 1 (src/main.rs:189-190):
     global histogram against the oracle and its closed form (digit 0, bin 1 =
@@ -526,6 +563,25 @@ def test_scan_modes_at_tile_edges(gpu_ctx, oracle, n):
     """Unique (u32, u64), exclusive scan and the fused edge-count scan at sizes
     around multiples of the scan kernels' 8192-element tile and of unique's
     16384-key (u32) and 8192-key (u64) tiles, against the oracle."""
+    check_scan_modes(gpu_ctx, oracle, n)
+
+
+@pytest.mark.parametrize("limit", [0, 1])
+@pytest.mark.parametrize("n", [16385, 3 * 16384 + 5, 300_001])
+def test_scan_modes_fallback(gpu_ctx, oracle, n, limit):
+    """The same scan modes with predecessor tiles reduced by the waiting
+    workgroup (every one at look-back limit 0, most at 1), over 2 to 37 tiles."""
+    import tc
+
+    L = tc.lib()
+    assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, limit) == 0
+    try:
+        check_scan_modes(gpu_ctx, oracle, n)
+    finally:
+        assert L.tc_set_lookback_spin_limit(gpu_ctx.handle, tc._lib.LOOKBACK_SPIN_DEFAULT) == 0
+
+
+def check_scan_modes(gpu_ctx, oracle, n):
     r = np.random.default_rng(n + 17)
     for dt in (np.uint32, np.uint64):
         keys = np.sort(r.integers(0, max(n // 3, 2), size=n, dtype=np.uint64).astype(dt))
