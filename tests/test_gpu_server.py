@@ -167,6 +167,32 @@ def test_large_batches_overlapping_on_two_streams(gpu_ctx, oracle, tclib):
         srv.close()
 
 
+def test_two_slots_under_overload_keep_their_throughput(gpu_ctx, tclib):
+    """Two slots with batches of up to 2^25 points under a 50k req/s Poisson
+    trace of 2^12 .. 2^22-point requests.  While the look-back kernels could
+    wait on workgroups that the other slot's kernel kept from starting, this
+    ran at 2.7-6.7 Gpoints/s (launches of 50-310 ms); with the waiting
+    workgroup computing the stalled tile it runs at about 17.  The floor of 8
+    Gpoints/s fails on that regression, far from timing noise."""
+    import tc
+
+    rng = np.random.default_rng(0xC0FFEE05)
+    sizes = np.round(np.exp(rng.uniform(np.log(2 ** 12), np.log(2 ** 22), 96))).astype(np.uint32)
+    pts = [dev(tclib.gen_uniform(int(s), 500 + i)) for i, s in enumerate(sizes)]
+    srv = tc.Server(gpu_ctx, n_streams=2, max_batch_points=1 << 25)
+    try:
+        srv.run_trace(np.arange(8) * 1e-4, pts[:8], sizes[:8])
+        k = 1500
+        t = np.cumsum(rng.exponential(1.0 / 50_000, k))
+        t -= t[0]
+        idx = rng.integers(0, len(pts), k)
+        lat = srv.run_trace(t, [pts[i] for i in idx], sizes[idx])
+        gpts = float(sizes[idx].sum()) / float(np.max(t + lat)) / 1e9
+        assert gpts > 8.0, f"{gpts:.2f} Gpoints/s"
+    finally:
+        srv.close()
+
+
 def test_63bit_served(gpu_ctx, oracle, tclib):
     import tc
 
