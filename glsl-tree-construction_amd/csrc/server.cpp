@@ -220,10 +220,10 @@ void scheduler_loop(tc_server* s) {
         if (progressed) continue;
         if (any_busy) {
             // A batch is running: poll its event again after a short sleep,
-            // or at once when a request or a release comes in.  The sleep
-            // gives the lock up; an unlock-yield-relock spin would win the
-            // mutex back before submitters and pollers could take it, and
-            // under load they starved for whole batches.
+            // or at once when a request or a release comes in.  The wait
+            // gives the lock up for the whole sleep; an unlock-yield-relock
+            // spin holds it almost all the time and can win it back before a
+            // waiting submitter or poller gets it (std::mutex is not fair).
             const uint64_t g = s->gen;
             s->cv_work.wait_for(lk, std::chrono::microseconds(20), [&] { return s->gen != g; });
         } else {
