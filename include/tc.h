@@ -236,7 +236,8 @@ int tc_sort_keys_u32_noalloc(tc_ctx* ctx, uint32_t* keys, uint32_t* keys_alt, ui
 
 /* ---- Unique: k_CountUnique, shader/unique.comp:31-45; host wrapper
  *      unique(), src/unique.rs:37.  out[0..U) = first key of each run of equal
- *      keys; U is written to *d_n_unique (device).  out must not alias sorted. */
+ *      keys; U is written to *d_n_unique (device).  out must not overlap sorted
+ *      (TC_ERR_INVALID). */
 int tc_unique_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out, uint32_t n,
                   uint32_t* d_n_unique, tc_stream stream);
 int tc_unique_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out, uint32_t n,
@@ -288,7 +289,8 @@ int tc_edge_count(tc_ctx* ctx, const uint8_t* prefix_n, const int32_t* parent, u
 /* ---- Exclusive scan: k_PartialSum, shader/partial_sum.comp:23-39 with the
  *      fix-up of src/main.rs:127-129; host wrapper partial_sum(),
  *      src/partial_sum.rs:43.  out has n+1 entries; out[n] = total.
- *      With d_n_keys the element count is (*d_n_keys - 1). */
+ *      With d_n_keys the element count is (*d_n_keys - 1).  out must not
+ *      overlap in (TC_ERR_INVALID). */
 int tc_exclusive_scan_u32(tc_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                           const uint32_t* d_n_keys, tc_stream stream);
 /* Edge count and exclusive scan in one pass (writes both arrays). */

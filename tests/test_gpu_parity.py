@@ -445,6 +445,17 @@ def test_invalid_arguments_rejected(gpu_ctx):
     assert L.tc_morton30(h, ctypes.c_void_p(8), ctypes.c_void_p(16), 4, 0.0, 1.0, None) == tc._lib.TC_ERR_INVALID
     k = empty(16, np.uint32)
     assert L.tc_sort_keys_u32(h, k.data_ptr(), k.data_ptr(), (1 << 30) + 1, None) == tc._lib.TC_ERR_TOO_LARGE
+    # Scans and unique read their input again when a look-back computes a
+    # stalled predecessor tile: outputs overlapping the input are refused.
+    buf, cnt = empty(64, np.uint32), empty(1, np.uint32)
+    p = buf.data_ptr()
+    assert L.tc_exclusive_scan_u32(h, p, p, 16, None, None) == tc._lib.TC_ERR_INVALID
+    assert L.tc_exclusive_scan_u32(h, p, p + 4 * 16, 16, None, None) == 0  # adjacent, not overlapping
+    assert L.tc_unique_u32(h, p, p + 4 * 8, 16, cnt.data_ptr(), None) == tc._lib.TC_ERR_INVALID
+    assert L.tc_unique_runs_u32(h, p, p + 4 * 32, p + 4 * 15, 16, cnt.data_ptr(), None) == tc._lib.TC_ERR_INVALID
+    import torch
+
+    torch.cuda.synchronize()
 
 
 @pytest.mark.parametrize("limit", [0, 1])
