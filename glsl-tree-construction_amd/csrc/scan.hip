@@ -716,21 +716,11 @@ int tc_launch_scan(tc_ctx* ctx, int mode, const void* in0, const void* in1, void
     return TC_OK;
 }
 
-namespace {
-// True if the byte ranges [a, a + an) and [b, b + bn) share an address.  The
-// scans read their inputs again when a look-back computes a stalled
-// predecessor tile, so an output may not overlap the input.
-bool overlaps(const void* a, size_t an, const void* b, size_t bn) {
-    const uintptr_t x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
-    return an && bn && x < y + bn && y < x + an;
-}
-}  // namespace
-
 extern "C" int tc_unique_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out, uint32_t n,
                              uint32_t* d_n_unique, tc_stream stream) {
     if (!ctx || !d_n_unique || (n && (!sorted || !out))) return TC_ERR_INVALID;
-    if (overlaps(sorted, (size_t)n * sizeof(uint32_t), out, (size_t)n * sizeof(uint32_t))) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (tc_overlaps(sorted, (size_t)n * sizeof(uint32_t), out, (size_t)n * sizeof(uint32_t))) return TC_ERR_INVALID;
     return tc_launch_scan(ctx, MODE_UNIQUE32, sorted, nullptr, out, nullptr, d_n_unique, n, nullptr, nullptr, 0,
                           nullptr, nullptr, as_stream(stream));
 }
@@ -738,8 +728,8 @@ extern "C" int tc_unique_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out,
 extern "C" int tc_unique_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out, uint32_t n,
                              uint32_t* d_n_unique, tc_stream stream) {
     if (!ctx || !d_n_unique || (n && (!sorted || !out))) return TC_ERR_INVALID;
-    if (overlaps(sorted, (size_t)n * sizeof(uint64_t), out, (size_t)n * sizeof(uint64_t))) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (tc_overlaps(sorted, (size_t)n * sizeof(uint64_t), out, (size_t)n * sizeof(uint64_t))) return TC_ERR_INVALID;
     return tc_launch_scan(ctx, MODE_UNIQUE64, sorted, nullptr, out, nullptr, d_n_unique, n, nullptr, nullptr, 0,
                           nullptr, nullptr, as_stream(stream));
 }
@@ -747,9 +737,9 @@ extern "C" int tc_unique_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out,
 extern "C" int tc_unique_runs_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t* out, uint32_t* run_first,
                                   uint32_t n, uint32_t* d_n_unique, tc_stream stream) {
     if (!ctx || !d_n_unique || !run_first || (n && (!sorted || !out))) return TC_ERR_INVALID;
-    if (overlaps(sorted, (size_t)n * sizeof(uint32_t), out, (size_t)n * sizeof(uint32_t))) return TC_ERR_INVALID;
-    if (overlaps(sorted, (size_t)n * sizeof(uint32_t), run_first, ((size_t)n + 1) * 4)) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (tc_overlaps(sorted, (size_t)n * sizeof(uint32_t), out, (size_t)n * sizeof(uint32_t))) return TC_ERR_INVALID;
+    if (tc_overlaps(sorted, (size_t)n * sizeof(uint32_t), run_first, ((size_t)n + 1) * 4)) return TC_ERR_INVALID;
     return tc_launch_scan(ctx, MODE_UNIQUE32, sorted, nullptr, out, run_first, d_n_unique, n, nullptr, nullptr, 0,
                           nullptr, nullptr, as_stream(stream));
 }
@@ -757,9 +747,9 @@ extern "C" int tc_unique_runs_u32(tc_ctx* ctx, const uint32_t* sorted, uint32_t*
 extern "C" int tc_unique_runs_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t* out, uint32_t* run_first,
                                   uint32_t n, uint32_t* d_n_unique, tc_stream stream) {
     if (!ctx || !d_n_unique || !run_first || (n && (!sorted || !out))) return TC_ERR_INVALID;
-    if (overlaps(sorted, (size_t)n * sizeof(uint64_t), out, (size_t)n * sizeof(uint64_t))) return TC_ERR_INVALID;
-    if (overlaps(sorted, (size_t)n * sizeof(uint64_t), run_first, ((size_t)n + 1) * 4)) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (tc_overlaps(sorted, (size_t)n * sizeof(uint64_t), out, (size_t)n * sizeof(uint64_t))) return TC_ERR_INVALID;
+    if (tc_overlaps(sorted, (size_t)n * sizeof(uint64_t), run_first, ((size_t)n + 1) * 4)) return TC_ERR_INVALID;
     return tc_launch_scan(ctx, MODE_UNIQUE64, sorted, nullptr, out, run_first, d_n_unique, n, nullptr, nullptr, 0,
                           nullptr, nullptr, as_stream(stream));
 }
@@ -767,8 +757,8 @@ extern "C" int tc_unique_runs_u64(tc_ctx* ctx, const uint64_t* sorted, uint64_t*
 extern "C" int tc_exclusive_scan_u32(tc_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                                      const uint32_t* d_n_keys, tc_stream stream) {
     if (!ctx || !out || (n && !in)) return TC_ERR_INVALID;
-    if (overlaps(in, (size_t)n * 4, out, ((size_t)n + 1) * 4)) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (tc_overlaps(in, (size_t)n * 4, out, ((size_t)n + 1) * 4)) return TC_ERR_INVALID;
     return tc_launch_scan(ctx, MODE_SCAN, in, nullptr, out, nullptr, nullptr, n, d_n_keys, nullptr, 0, nullptr,
                           nullptr, as_stream(stream));
 }

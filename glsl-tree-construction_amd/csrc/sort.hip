@@ -659,6 +659,8 @@ int sort_impl(tc_ctx* ctx, K* keys, uint32_t* vals, K* alt, uint32_t* valt, uint
     if (!ctx || (n && (!keys || !alt))) return TC_ERR_INVALID;
     if (vals && !valt) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (tc_overlaps(keys, (size_t)n * sizeof(K), alt, (size_t)n * sizeof(K))) return TC_ERR_INVALID;
+    if (vals && tc_overlaps(vals, (size_t)n * 4, valt, (size_t)n * 4)) return TC_ERR_INVALID;
     if (n == 0) return TC_OK;
     if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
     const int kb = pts ? morton_key_bits((int)sizeof(K)) : plain_key_bits((int)sizeof(K));
@@ -686,6 +688,8 @@ int binning_pass_impl(tc_ctx* ctx, const K* sk, K* dk, const uint32_t* sv, uint3
     if ((sv == nullptr) != (dv == nullptr)) return TC_ERR_INVALID;
     if (shift % 8 != 0 || shift >= 8 * sizeof(K)) return TC_ERR_INVALID;
     if (n > TC_MAX_SORT_N) return TC_ERR_TOO_LARGE;
+    if (tc_overlaps(sk, (size_t)n * sizeof(K), dk, (size_t)n * sizeof(K))) return TC_ERR_INVALID;
+    if (sv && tc_overlaps(sv, (size_t)n * 4, dv, (size_t)n * 4)) return TC_ERR_INVALID;
     if (n == 0) return TC_OK;
     const SortShape sh = eight_bit_shape(tc_sort_shape(ctx, (int)sizeof(K), sv != nullptr, n));
     const uint32_t tiles = ceil_div_u32(n, (uint32_t)(sh.block * sh.kpt));
@@ -825,6 +829,7 @@ int tc_noalloc_init(tc_ctx* ctx) {
 extern "C" int tc_sort_keys_u32_noalloc(tc_ctx* ctx, uint32_t* keys, uint32_t* alt, uint32_t n, tc_stream stream) {
     if (!ctx || (n && (!keys || !alt))) return TC_ERR_INVALID;
     if (n > NA_MAX_N) return TC_ERR_TOO_LARGE;
+    if (tc_overlaps(keys, (size_t)n * 4, alt, (size_t)n * 4)) return TC_ERR_INVALID;
     if (n == 0) return TC_OK;
     if (reinterpret_cast<uintptr_t>(keys) & 15u) return TC_ERR_INVALID;
     uint32_t* base = ctx->noalloc_scratch;

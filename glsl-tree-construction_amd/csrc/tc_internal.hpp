@@ -11,6 +11,15 @@
 
 #include "../../include/tc.h"
 
+// True if the byte ranges [a, a + an) and [b, b + bn) share an address.  The
+// sort and scan kernels read their input again when a look-back computes a
+// stalled predecessor tile (and the binning passes scatter into the other
+// buffer), so entry points refuse outputs that overlap an input.
+inline bool tc_overlaps(const void* a, size_t an, const void* b, size_t bn) {
+    const uintptr_t x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
+    return an && bn && x < y + bn && y < x + an;
+}
+
 #define TC_HIP(call)                                   \
     do {                                               \
         hipError_t e_ = (call);                        \

@@ -453,6 +453,7 @@ def test_invalid_arguments_rejected(gpu_ctx):
     assert L.tc_exclusive_scan_u32(h, p, p + 4 * 16, 16, None, None) == 0  # adjacent, not overlapping
     assert L.tc_unique_u32(h, p, p + 4 * 8, 16, cnt.data_ptr(), None) == tc._lib.TC_ERR_INVALID
     assert L.tc_unique_runs_u32(h, p, p + 4 * 32, p + 4 * 15, 16, cnt.data_ptr(), None) == tc._lib.TC_ERR_INVALID
+    assert L.tc_sort_keys_u32(h, p, p + 4 * 8, 16, None) == tc._lib.TC_ERR_INVALID
     import torch
 
     torch.cuda.synchronize()
